@@ -1,0 +1,272 @@
+"""Synthetic case generators for the benchmark configurations.
+
+The reference hard-codes one coastal-aquifer scenario in its generators
+(boundary.f90:12-606, internal.f90:10-445, setup.f90:10-150).  The
+configurations measured here are different scenarios built on the same
+conventions:
+
+* three wall layers at the fluid spacing, the innermost layer "fixed"
+  (tid 1) and the outer two "ghost" (tid 2)            -- boundary.f90:47-86
+* fluid particles on a regular lattice, mass rho*dx^2, volume dx^2,
+  hydrostatic initial pressure                          -- internal.f90:37-91
+* static porous-solid particles of bulk density 1800 whose kernel sum gives
+  the porosity                                          -- boundary.f90:523-593
+* the scalar is stored as the reference does, c_ref in [-0.5, 0.5]
+  (main.f90:169-179, output.f90:203): a reported concentration c in {0, 1}
+  maps to c_ref = c - 0.5, so densityupdate gives rho0 = 1000 + 25 c
+  (scalar.f90:414, setup.f90:19).
+* the cell grid has one halo ring; interior cell (ci, cj) covers
+  x in [(cj-1)*4h, cj*4h), y in [(celly-2-ci)*4h, (celly-1-ci)*4h)
+  (setup.f90:120-123), so every particle is placed at x, y >= 0.
+
+Each generator returns a `Case`: the run parameters (the fields of the C
+struct `isph_params`), the particle SoA arrays and the solid SoA arrays.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+TID_FIXED, TID_GHOST, TID_FLUID = 1, 2, 3
+F_BUFFER, F_FREE, F_VICINITY = 1, 2, 4
+
+RHO = 1000.0
+REL_DEN = 1.025
+G = -9.81
+BULKDEN = 1800.0
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    params: dict
+    parts: dict       # x y u v p m rho rho0 vol0 c (f64), tid (i32), flags (u8)
+    solids: dict      # x y m rho (f64)
+    steps: int
+    meta: dict
+
+    @property
+    def n(self) -> int:
+        return int(self.parts["x"].shape[0])
+
+    @property
+    def ns(self) -> int:
+        return int(self.solids["x"].shape[0])
+
+
+def default_params(dx: float, h_fac: float = 1.3) -> dict:
+    """Run parameters with the reference's constants (initialize.f90:21-83,
+    porous.f90:111, turbulence.f90:16) for a lattice spacing dx."""
+    h = h_fac * dx
+    return dict(
+        cellx=0, celly=0, fixed_dt=1, precision=0, max_iter=20000, max_outer=20,
+        h=h, dl=dx, lam=0.0010 * (h * h), rho=RHO, rel_den=REL_DEN, rhomin=1000.0,
+        g=G, mu=0.0010, por=0.380, d50=0.001020, lamk=600.0, csh=0.50,
+        maxshift=0.1, fs_thresh=0.80 * 2.0, sig1=0.20, tschmidt=1.0,
+        xlcutoff=-1.0e30, ytcutoff=-1.0e30, buf_xmin=1.0e30, buf_xmax=-1.0e30,
+        dt=1.0e-4, tol=1.0e-10, inner_tol=1.0e-4,
+    )
+
+
+def _grid(params: dict, xmax: float, ymax: float) -> None:
+    cs = 4.0 * params["h"]
+    params["cellx"] = int(math.ceil(xmax / cs)) + 3
+    params["celly"] = int(math.ceil(ymax / cs)) + 3
+
+
+def _walls_box(x0: float, x1: float, y0: float, ytop: float, dx: float):
+    """Three wall layers below [x0,x1] and outside x0 / x1 up to ytop.
+    Returns (x, y, tid) with the innermost layer tid 1."""
+    xs, ys, ts = [], [], []
+    nx = int(round((x1 - x0) / dx))
+    ny = int(round((ytop - y0) / dx))
+    for k in range(3):
+        t = TID_FIXED if k == 0 else TID_GHOST
+        # bottom, including the corner columns of the side walls
+        xb = x0 + (np.arange(-k - 1, nx + k + 1) + 0.5) * dx
+        xs.append(xb)
+        ys.append(np.full(xb.shape, y0 - (k + 0.5) * dx))
+        ts.append(np.full(xb.shape, t))
+        yv = y0 + (np.arange(-k, ny) + 0.5) * dx
+        for xw in (x0 - (k + 0.5) * dx, x1 + (k + 0.5) * dx):
+            xs.append(np.full(yv.shape, xw))
+            ys.append(yv)
+            ts.append(np.full(yv.shape, t))
+    return np.concatenate(xs), np.concatenate(ys), np.concatenate(ts).astype(np.int32)
+
+
+def _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, depth_top, sx=None, sy=None,
+              steps=1, meta=None) -> Case:
+    n_f, n_w = fx.size, wx.size
+    x = np.concatenate([wx, fx])
+    y = np.concatenate([wy, fy])
+    tid = np.concatenate([wt, np.full(n_f, TID_FLUID, np.int32)]).astype(np.int32)
+    c_ref = np.concatenate([np.full(n_w, -0.5), fc - 0.5])
+    rhomax = params["rho"] * params["rel_den"]
+    rho0 = 0.5 * (rhomax + params["rhomin"]) + c_ref * (rhomax - params["rhomin"])
+    rho0[:n_w] = RHO
+    vol0 = np.full(x.size, dx * dx)
+    m = rho0 * vol0
+    p = np.maximum(RHO * abs(G) * (depth_top - y), 0.0)
+    parts = dict(
+        x=x, y=y, u=np.zeros_like(x), v=np.zeros_like(x), p=p, m=m, rho=rho0.copy(),
+        rho0=rho0, vol0=vol0, c=c_ref, tid=tid, flags=np.zeros(x.size, np.uint8),
+    )
+    if sx is None:
+        sx = np.zeros(0)
+        sy = np.zeros(0)
+    vs = dx * dx
+    solids = dict(x=sx.astype(np.float64), y=sy.astype(np.float64),
+                  m=np.full(sx.size, BULKDEN * vs), rho=np.full(sx.size, BULKDEN))
+    return Case(name=name, params=params, parts=parts, solids=solids, steps=steps,
+                meta=meta or {})
+
+
+def dam_break(dx: float = 0.005, jitter: float = 0.0, seed: int = 1234,
+              tank=(2.0, 0.6), column=(0.5, 0.25), toe: float = 1.0,
+              slope: float = 0.1, steps: int = 200) -> Case:
+    """Dam break onto a 1:10 impermeable slope (FP64 configuration, and with a
+    seeded uniform +-jitter*dx on fluid positions the precision-form one).
+    Tank 2.0 x 0.6 m, water column 0.5 x 0.25 m, h = 1.3 dx, dt = 1e-4 s."""
+    params = default_params(dx)
+    params.update(dt=1.0e-4, tol=1.0e-10)
+    off = 3.0 * dx + 0.5 * dx            # keeps the walls at x, y >= 0
+    L, H = tank
+    wx, wy, wt = _walls_box(off, off + L, off, off + H, dx)
+    # 1:10 slope from the toe to the right wall: three layers under the line
+    xs_ = off + toe + (np.arange(int(round((L - toe) / dx))) + 0.5) * dx
+    sl_x, sl_y, sl_t = [], [], []
+    for k in range(3):
+        yk = off + (xs_ - off - toe) * slope - (k + 0.5) * dx
+        keep = yk > off + 0.25 * dx
+        sl_x.append(xs_[keep])
+        sl_y.append(yk[keep])
+        sl_t.append(np.full(int(keep.sum()), TID_FIXED if k == 0 else TID_GHOST, np.int32))
+    wx = np.concatenate([wx] + sl_x)
+    wy = np.concatenate([wy] + sl_y)
+    wt = np.concatenate([wt] + sl_t)
+    nx, ny = int(round(column[0] / dx)), int(round(column[1] / dx))
+    gx, gy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    fx = off + (gx.ravel() + 0.5) * dx
+    fy = off + (gy.ravel() + 0.5) * dx
+    if jitter:
+        rng = np.random.default_rng(seed)
+        fx = fx + rng.uniform(-jitter * dx, jitter * dx, fx.size)
+        fy = fy + rng.uniform(-jitter * dx, jitter * dx, fy.size)
+    fc = np.zeros(fx.size)
+    _grid(params, off + L + 4 * dx, off + H + 4 * dx)
+    name = "dam_break_jitter" if jitter else "dam_break"
+    return _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, off + column[1],
+                     steps=steps, meta=dict(dx=dx, jitter=jitter, seed=seed))
+
+
+def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
+                 tank_h: float = 0.6, toe: float = 6.0, slope: float = 0.1,
+                 porosity: float = 0.4, d50: float = 0.01,
+                 patch=(0.2, 0.1), dt: float = 5.0e-5, steps: int = 2000,
+                 wave_h: float = 0.05, wave_t: float = 1.5, grade_h: float = 8.0,
+                 name: str = "porous_flume") -> Case:
+    """Wave-runup flume on a porous beach (the 650k-particle configuration at
+    the defaults: 4000 x 160 fluid lattice, 3 wall layers).  The 1:10 beach
+    under the line y = (x - toe)*slope is filled with static solid particles
+    at the fluid spacing; the fluid lattice fills the pores as in the
+    reference's seepage case.  A scalar patch of c = 1 (0.2 m x 0.1 m) sits at
+    the toe.  The wavemaker paddle is held at rest: the piston amplitude
+    (wave_h, wave_t) is recorded in meta but not applied.
+
+    The beach surface is graded: solid volume ramps linearly from 0 at the
+    beach surface to full at a depth of grade_h*h, so the porosity goes from 1
+    to `porosity` over that layer.  With a sharp porosity jump the reference's
+    pressure projection amplifies the velocity divergence at the interface by
+    a dt-independent factor each step and the run diverges within a few
+    steps; a graded layer keeps it bounded."""
+    params = default_params(dx)
+    params.update(dt=dt, por=porosity, d50=d50, tol=1.0e-10)
+    off = 3.0 * dx + 0.5 * dx
+    wx, wy, wt = _walls_box(off, off + length, off, off + tank_h, dx)
+    nx, ny = int(round(length / dx)), int(round(depth / dx))
+    gx, gy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    fx = off + (gx.ravel() + 0.5) * dx
+    fy = off + (gy.ravel() + 0.5) * dx
+    lx = fx - off
+    ly = fy - off
+    fc = ((np.abs(lx - toe) <= 0.5 * patch[0]) & (ly <= patch[1])).astype(np.float64)
+    # porous beach solids
+    sxi = np.arange(int(math.floor(toe / dx)), nx)
+    sgx, sgy = np.meshgrid(sxi, np.arange(ny + 40), indexing="ij")
+    sx = (sgx.ravel() + 0.5) * dx
+    sy = (sgy.ravel() + 0.5) * dx
+    below = (sx - toe) * slope - sy
+    keep = below > 0.0
+    weight = np.clip(below[keep] / (grade_h * params["h"]), 0.0, 1.0) if grade_h > 0 \
+        else np.ones(int(keep.sum()))
+    sx = off + sx[keep]
+    sy = off + sy[keep]
+    _grid(params, off + length + 4 * dx, off + tank_h + 4 * dx)
+    c = _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, off + depth, sx, sy,
+                  steps=steps, meta=dict(dx=dx, length=length, depth=depth, toe=toe,
+                                         slope=slope, porosity=porosity, d50=d50,
+                                         wave_h=wave_h, wave_t=wave_t, grade_h=grade_h))
+    c.solids["m"] = c.solids["m"] * weight
+    return c
+
+
+def scenario_mix(n_scen: int = 64, seed: int = 2024, n_min: int = 20_000,
+                 n_max: int = 300_000, log_uniform: bool = False, steps: int = 1000):
+    """Independent porous-beach scenarios of uneven particle count with
+    parameters drawn as the batched configuration states: slope 1:8-1:20,
+    porosity 0.3-0.5, d50 0.005-0.02 m, H 0.02-0.08 m, T 1.0-2.5 s.  Each
+    flume is 6 m x 0.3 m; the lattice spacing is chosen so the fluid count
+    matches the drawn N."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n_scen):
+        if log_uniform:
+            n = int(math.exp(rng.uniform(math.log(n_min), math.log(n_max))))
+        else:
+            n = int(rng.integers(n_min, n_max + 1))
+        slope = 1.0 / rng.uniform(8.0, 20.0)
+        por = rng.uniform(0.3, 0.5)
+        d50 = rng.uniform(0.005, 0.02)
+        wh = rng.uniform(0.02, 0.08)
+        wt = rng.uniform(1.0, 2.5)
+        length, depth = 6.0, 0.3
+        dx = math.sqrt(length * depth / n)
+        c = porous_flume(dx=dx, length=length, depth=depth, tank_h=0.45,
+                         toe=length - depth / slope if depth / slope < length else 0.5,
+                         slope=slope, porosity=por, d50=d50, dt=0.2 * dx / 10.0,
+                         steps=steps, wave_h=wh, wave_t=wt, name=f"scenario{k}")
+        c.params["tol"] = 1.0e-8
+        cases.append(c)
+    return cases
+
+
+def tiny_box(nx: int = 12, ny: int = 10, dx: float = 0.01, jitter: float = 0.05,
+             seed: int = 7, solids: bool = True, patch: bool = True) -> Case:
+    """A few-hundred-particle tank for oracle-speed parity tests: jittered
+    fluid, three wall layers, an optional porous corner and scalar patch."""
+    params = default_params(dx)
+    params.update(dt=2.0e-4, tol=1.0e-12, por=0.4, d50=0.01)
+    off = 3.0 * dx + 0.5 * dx
+    L, H = (nx + 4) * dx, (ny + 6) * dx
+    wx, wy, wt = _walls_box(off, off + L, off, off + H, dx)
+    gx, gy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    rng = np.random.default_rng(seed)
+    fx = off + (gx.ravel() + 0.5) * dx + rng.uniform(-jitter, jitter, gx.size) * dx
+    fy = off + (gy.ravel() + 0.5) * dx + rng.uniform(-jitter, jitter, gy.size) * dx
+    fc = ((gx.ravel() < nx // 3) & (gy.ravel() < ny // 3)).astype(np.float64) if patch \
+        else np.zeros(fx.size)
+    sx = sy = None
+    if solids:
+        sgx, sgy = np.meshgrid(np.arange(nx // 2, nx + 4), np.arange(ny // 2), indexing="ij")
+        sx = off + (sgx.ravel() + 0.5) * dx
+        sy = off + (sgy.ravel() + 0.5) * dx
+    _grid(params, off + L + 4 * dx, off + H + 4 * dx)
+    c = _assemble(params, "tiny_box", fx, fy, fc, wx, wy, wt, dx, off + ny * dx, sx, sy,
+                  steps=3, meta=dict(nx=nx, ny=ny, dx=dx, jitter=jitter, seed=seed))
+    # a little motion so every term of the pipeline is exercised
+    c.parts["u"][wx.size:] = 0.05 * np.sin(fy * 40.0)
+    c.parts["v"][wx.size:] = 0.03 * np.cos(fx * 30.0)
+    return c

@@ -1,0 +1,200 @@
+// isph_device.h -- device data layout and the per-particle math of the ISPH
+// pipeline for gfx950.
+//
+// Layout in HBM (one context = one scenario):
+//   * particle state as structure-of-arrays in cell-sorted order; a particle's
+//     index is its PPE row (the reference's matid - 1 after resetid,
+//     isph.f90:88-113);
+//   * neighbour lists and PPE rows in ELL form, slot-major
+//     (a[k*cap + i] for slot k of particle i), so that a wavefront reading
+//     slot k of 64 consecutive particles touches one contiguous run;
+//   * cell starts for the cell-linked list, (cell, transfer-class) bins for
+//     the rebin.
+// Every expression below is evaluated in the order it is written in the
+// reference Fortran (files cited per function); the library is compiled with
+// -ffp-contract=off so no FMA is formed and every rounding happens where the
+// Fortran rounds.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NBR_CAP 64
+#define ROW_CAP (NBR_CAP + 2)
+#define RED_BLOCK 256
+#define NCLS 10     // transfer classes per destination cell
+#define CLS_DROP 15
+
+struct DevParams {
+  int cellx, celly;
+  double h, dl, lam, rho, rel_den, rhomin, g, mu, por, d50, lamk, csh, maxshift;
+  double fs_thresh, sig1, tschmidt, xlcutoff, ytcutoff, buf_xmin, buf_xmax;
+  double tol, inner_tol;
+  int fixed_dt, max_iter;
+  // constants derived on the host with the same expressions as the reference
+  double cs;        // incr*h1 = 4h (setup.f90:120-123)
+  double k1, k2;    // 10/(28*3.14 h^2), 1/(3*3.14 h^2), real4 products (kernel.f90:61-62)
+  double two_h, lim2, wo;
+};
+
+// Device-resident scalars (the module variables t, dt, dtsol, solsteps,
+// umax, numax of initialize.f90:21-60) plus solver state.
+struct DevScalars {
+  double t, dt, dtsol, umax, numax;
+  unsigned long long umax_bits, numax_bits;  // pending maxima from comp_vel
+  int solsteps, n, n_hi_unused, err;
+  // BiCGStab state
+  double rho, alpha, omega, beta, rr, bnorm, stop, rel;
+  int iters, done, conv, brk;
+  unsigned int counter[4];
+};
+
+// error bits in DevScalars::err
+#define ERR_NBR_CAP 1
+#define ERR_DIAG 2
+
+struct Dev {
+  DevParams P;
+  DevScalars* sc;
+  int cap;
+  // gathered particle state (ping-pong buffers swapped by the host)
+  double *x, *y, *u, *v, *us, *vs, *xs, *ys, *p, *m, *rho, *rho0, *vol0, *c;
+  uint8_t *tid, *flags, *vic;
+  int32_t* cell;
+  double *x2, *y2, *u2, *v2, *us2, *vs2, *xs2, *ys2, *p2, *m2, *rho2, *rho02, *vol02, *c2;
+  uint8_t *tid2, *flags2, *vic2;
+  int32_t* cell2;
+  // derived properties
+  double *por, *resist, *gradvx, *cf1, *cf2, *cf3, *cf4, *nut, *tke, *varts, *cdiff, *vel;
+  double *xnb, *ynb;
+  int32_t *ncnt, *nbr;
+  int32_t *cstart, *bincnt, *binstart, *pbin, *prank, *scan_tmp;
+  // solids
+  double *sx, *sy, *sm, *srho;
+  int32_t* sstart;
+  // PPE rows and solver vectors
+  int32_t *alen, *acol;
+  double *aval, *b, *sol;
+  float* aval32;
+  double *r, *rh, *pp, *t1, *sv, *t2, *part;
+  float *r32, *rh32, *pp32, *t132, *sv32, *t232, *d32;
+};
+
+// ---------------------------------------------------------------------
+// kernel.f90 restatement (device)
+// ---------------------------------------------------------------------
+// Wab, kernel.f90:34-66 (hWab, kernel.f90:164-184, is identical).
+__device__ __forceinline__ double d_wab(double r, const DevParams& P) {
+  const double h = P.h;
+  double t1, t2;
+  if (r < h) {
+    t1 = (4.0 - ((6.0 * (r * r)) / (h * h))) + ((3.0 * ((r * r) * r)) / ((h * h) * h));
+    const double q = r / h;
+    t2 = (((q * q) * q) - ((6.0 * r) / h)) + 6.0;
+  } else {
+    const double a = 2.0 - r / h;
+    t1 = (a * a) * a;
+    t2 = t1;
+  }
+  return ((0.1 * t1) * P.k1) + (((1.0 - 0.1) * t2) * P.k2);
+}
+
+// Wabx / Waby (kernel.f90:94-162) and hWabx / hWaby (kernel.f90:204-248) for
+// the pair (p1 = j, p2 = i): d_ij = coordinate of i minus coordinate of j.
+__device__ __forceinline__ double d_wabd(double d_ij, double r, const DevParams& P) {
+  const double h = P.h;
+  double t1, t2;
+  if (r < h) {
+    t1 = (((9.0 * r) / ((h * h) * h)) - (12.0 / (h * h))) * d_ij;
+    t2 = (((3.0 * r) / ((h * h) * h)) - (6.0 / (r * h))) * d_ij;
+  } else {
+    const double a = 2.0 - r / h;
+    const double f = (3.0 / (r * h)) * (a * a);
+    t1 = f * (-d_ij);
+    t2 = t1;
+  }
+  return ((0.1 * t1) * P.k1) + (((1.0 - 0.1) * t2) * P.k2);
+}
+
+// kernel value and gradient pair for one neighbour (both components share r)
+struct Klt {
+  double r, k1, k2;
+};
+__device__ __forceinline__ Klt d_klt(double dx, double dy, const DevParams& P) {
+  Klt o;
+  o.r = sqrt((dx * dx) + (dy * dy));  // dist, functions.f90:28-42
+  o.k1 = d_wabd(dx, o.r, P);
+  o.k2 = d_wabd(dy, o.r, P);
+  return o;
+}
+
+// vis, porous.f90:12-30 / integrator.f90:12-30
+__device__ __forceinline__ double d_vis(double oden) {
+  const double sp = oden - 1000.0;
+  return ((1.002 + (0.001652 * sp)) + (0.0000083 * (sp * sp))) * 0.001;
+}
+
+__device__ __forceinline__ void cell_bounds(int key, const DevParams& P, int& ci, int& cj,
+                                            double& xl, double& xr, double& yb, double& yt) {
+  cj = key / P.celly;
+  ci = key - cj * P.celly;
+  xl = (double)(cj - 1) * P.cs;
+  xr = (double)(cj)*P.cs;
+  yb = (double)(P.celly - 2 - ci) * P.cs;
+  yt = (double)(P.celly - 1 - ci) * P.cs;
+}
+
+// quadrant of the particle in its cell (search.f90:328-388, porous.f90:133-178):
+// returns 0 LL, 1 LR, 2 UR, 3 UL
+__device__ __forceinline__ int quadrant(double x, double y, double xl, double xr, double yb,
+                                        double yt) {
+  const double xmid = xl + fabs((xl - xr) / 2);
+  const double ymid = yb + fabs((yt - yb) / 2);
+  if (x <= xmid && y <= ymid) return 0;
+  if (x > xmid && y <= ymid) return 1;
+  if (x > xmid && y > ymid) return 2;
+  return 3;
+}
+
+// ---------------------------------------------------------------------
+// deterministic block reduction: halving tree v[t] += v[t+s], s = 128..1.
+// The first two levels go through LDS, the last six through wavefront
+// shuffles inside wave 0; the result is identical to the LDS tree.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double block_tree(double val, double* lds) {
+  const int t = threadIdx.x;
+  lds[t] = val;
+  __syncthreads();
+  if (t < 128) lds[t] = lds[t] + lds[t + 128];
+  __syncthreads();
+  double v = 0.0;
+  if (t < 64) {
+    v = lds[t] + lds[t + 64];
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v = v + __shfl_down(v, s, 64);
+  }
+  __syncthreads();
+  return v;  // valid in thread 0
+}
+
+// Final reduction of nb block partials in the fixed order: lane t adds
+// partials t, t+256, ... sequentially, then the halving tree.
+__device__ __forceinline__ double final_tree(const volatile double* part, int nb, double* lds) {
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nb; k += RED_BLOCK) acc = acc + part[k];
+  return block_tree(acc, lds);
+}
+
+// "last block finishes": returns true in every thread of the block that
+// observed all other blocks' partials (one counter per reduction site).
+__device__ __forceinline__ bool last_block(unsigned int* counter) {
+  __shared__ bool amlast;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int prev = atomicAdd(counter, 1u);
+    amlast = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (amlast) __threadfence();
+  return amlast;
+}

@@ -1,0 +1,832 @@
+// isph_kernels.h -- HIP kernels of the per-step ISPH pipeline (gfx950).
+//
+// One thread per particle (or PPE row) unless stated; blocks of 256 threads
+// (four wave64s).  Kernels read the live particle count from the device
+// scalars, so launch shapes depend only on a host-side upper bound and a
+// step can be replayed without host round trips.
+#pragma once
+#include "isph_device.h"
+
+#define TID_FIXED 1
+#define TID_GHOST 2
+#define TID_FLUID 3
+#define F_BUFFER 1
+#define F_FREE 2
+
+#define PIDX()                                   \
+  const int i = blockIdx.x * blockDim.x + threadIdx.x; \
+  const int n = D.sc->n;
+
+__global__ void k_fill_i32(int32_t* a, int len, int val) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) a[i] = val;
+}
+
+// ---------------------------------------------------------------------
+// projection, isph.f90:17-47
+// ---------------------------------------------------------------------
+__global__ void k_projection(Dev D) {
+  PIDX();
+  if (i >= n || D.tid[i] != TID_FLUID) return;
+  const double dt = D.sc->dt;
+  D.xs[i] = D.x[i];
+  D.x[i] = D.x[i] + ((dt * D.u[i]) / D.por[i]);
+  D.ys[i] = D.y[i];
+  D.y[i] = D.y[i] + ((dt * D.v[i]) / D.por[i]);
+}
+
+// ---------------------------------------------------------------------
+// rebin = cellshift / cellshiftalt (integrator.f90:33-392).
+// A particle moves at most one cell per call, so its destination is fixed by
+// its transfer class: 0 wall, 1 stay, then tn ts tw te tse tne tsw tnw in the
+// order the receiving cell appends them (integrator.f90:324-386).  Sorting by
+// the (destination cell, class) bin, stably in the current order, gives the
+// reference's slot order exactly: one counting-sort pass with prefix-summed
+// bin starts (the bins of a cell are consecutive, so its first bin start is
+// the cell start).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int dest_offset_ci(int cls) {
+  // row offset of the destination cell relative to the source cell
+  switch (cls) {
+    case 2: return -1;  // tn: north = up = row - 1
+    case 3: return +1;  // ts
+    case 6: return +1;  // tse
+    case 7: return -1;  // tne
+    case 8: return +1;  // tsw
+    case 9: return -1;  // tnw
+    default: return 0;
+  }
+}
+__device__ __forceinline__ int dest_offset_cj(int cls) {
+  switch (cls) {
+    case 4: return -1;  // tw
+    case 5: return +1;  // te
+    case 6: return +1;  // tse
+    case 7: return +1;  // tne
+    case 8: return -1;  // tsw
+    case 9: return -1;  // tnw
+    default: return 0;
+  }
+}
+
+__global__ void k_rebin_classify(Dev D, int alt) {
+  PIDX();
+  if (i >= n) return;
+  const DevParams& P = D.P;
+  int cls;
+  const int key = D.cell[i];
+  if (D.tid[i] != TID_FLUID) {
+    cls = 0;
+  } else {
+    int ci, cj;
+    double xl, xr, yb, yt;
+    cell_bounds(key, P, ci, cj, xl, xr, yb, yt);
+    const double x1 = D.x[i], y1 = D.y[i];
+    cls = CLS_DROP;
+    if ((x1 >= xl) && (x1 < xr)) {
+      if ((y1 >= yb) && (y1 < yt)) cls = 1;
+      else if (y1 < yb) cls = 3;
+      else if (y1 >= yt) cls = 2;
+    } else if (x1 >= xr) {
+      if (y1 >= yt) cls = 7;
+      else if ((y1 >= yb) && (y1 < yt)) cls = 5;
+      else if (y1 < yb) cls = 6;
+    } else if (x1 < xl) {
+      if (y1 >= yt) cls = 9;
+      else if ((y1 >= yb) && (y1 < yt)) cls = 4;
+      else if (y1 < yb) cls = 8;
+    }
+    if (alt && cls != CLS_DROP) {
+      const bool buf = (D.flags[i] & F_BUFFER) != 0;
+      if (!((x1 > P.xlcutoff) || buf || ((x1 <= P.xlcutoff) && (y1 <= P.ytcutoff)))) cls = CLS_DROP;
+    }
+    if (cls != CLS_DROP) {
+      const int dci = ci + dest_offset_ci(cls), dcj = cj + dest_offset_cj(cls);
+      // only interior cells are rebuilt (integrator.f90:320-321)
+      if (dci < 1 || dci > P.celly - 2 || dcj < 1 || dcj > P.cellx - 2) cls = CLS_DROP;
+    }
+  }
+  int bin = -1;
+  if (cls != CLS_DROP) {
+    const int ci = key % P.celly, cj = key / P.celly;
+    const int dkey = (cj + dest_offset_cj(cls)) * P.celly + (ci + dest_offset_ci(cls));
+    bin = dkey * NCLS + cls;
+    atomicAdd(&D.bincnt[bin], 1);
+  }
+  D.pbin[i] = bin;
+}
+
+// rank of a particle among the earlier particles of its source cell with the
+// same bin (all particles of one bin come from one source cell)
+__global__ void k_rebin_rank(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  const int bin = D.pbin[i];
+  if (bin < 0) return;
+  const int s0 = D.cstart[D.cell[i]];
+  int rank = 0;
+  for (int j = s0; j < i; j++) rank += (D.pbin[j] == bin);
+  D.prank[i] = rank;
+}
+
+// scatter every gathered field into the new order (ping-pong buffers)
+__global__ void k_rebin_scatter(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  const int bin = D.pbin[i];
+  if (bin < 0) return;
+  const int d = D.binstart[bin] + D.prank[i];
+  D.x2[d] = D.x[i]; D.y2[d] = D.y[i]; D.u2[d] = D.u[i]; D.v2[d] = D.v[i];
+  D.us2[d] = D.us[i]; D.vs2[d] = D.vs[i]; D.xs2[d] = D.xs[i]; D.ys2[d] = D.ys[i];
+  D.p2[d] = D.p[i]; D.m2[d] = D.m[i]; D.rho2[d] = D.rho[i]; D.rho02[d] = D.rho0[i];
+  D.vol02[d] = D.vol0[i]; D.c2[d] = D.c[i];
+  D.tid2[d] = D.tid[i]; D.flags2[d] = D.flags[i]; D.vic2[d] = D.vic[i];
+  D.cell2[d] = bin / NCLS;
+}
+
+__global__ void k_rebin_finish(Dev D, int ncell) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= ncell) D.cstart[c] = D.binstart[c * NCLS];
+  if (c == 0) D.sc->n = D.binstart[ncell * NCLS];
+}
+
+// ---------------------------------------------------------------------
+// exclusive scan of int32 (bins -> bin starts), three passes
+// ---------------------------------------------------------------------
+#define SCAN_ITEMS 4
+#define SCAN_TILE (256 * SCAN_ITEMS)
+__global__ void k_scan_tiles(const int32_t* in, int32_t* out, int32_t* tile_sums, int len) {
+  __shared__ int32_t s[256];
+  const int t = threadIdx.x;
+  const int base = blockIdx.x * SCAN_TILE + t * SCAN_ITEMS;
+  int32_t v[SCAN_ITEMS], sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    v[k] = (base + k < len) ? in[base + k] : 0;
+    sum += v[k];
+  }
+  s[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int32_t a = (t >= off) ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += a;
+    __syncthreads();
+  }
+  int32_t run = s[t] - sum;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < len) out[base + k] = run;
+    run += v[k];
+  }
+  if (t == 255) tile_sums[blockIdx.x] = s[255];
+}
+
+__global__ void k_scan_sums(int32_t* tile_sums, int ntiles, int32_t* total_out) {
+  __shared__ int32_t s[256];
+  const int t = threadIdx.x;
+  int32_t carry = 0;
+  for (int base = 0; base < ntiles; base += 256) {
+    int32_t v = (base + t < ntiles) ? tile_sums[base + t] : 0;
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      int32_t a = (t >= off) ? s[t - off] : 0;
+      __syncthreads();
+      s[t] += a;
+      __syncthreads();
+    }
+    if (base + t < ntiles) tile_sums[base + t] = carry + s[t] - v;
+    int32_t tot = s[255];
+    __syncthreads();
+    carry += tot;
+  }
+  if (t == 0 && total_out) *total_out = carry;
+}
+
+__global__ void k_scan_add(int32_t* out, const int32_t* tile_sums, int len) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < len) out[idx] += tile_sums[idx / SCAN_TILE];
+}
+
+// ---------------------------------------------------------------------
+// neighbour / nsearch2, search.f90:13-399
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_neighbour(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  const DevParams& P = D.P;
+  const double xi = D.x[i], yi = D.y[i];
+  int ci, cj;
+  double xl, xr, yb, yt;
+  cell_bounds(D.cell[i], P, ci, cj, xl, xr, yb, yt);
+  const int q = quadrant(xi, yi, xl, xr, yb, yt);
+  const int r0 = ci - ((q == 2 || q == 3) ? 1 : 0);
+  const int c0 = cj - ((q == 0 || q == 3) ? 1 : 0);
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  int cnt = 0;
+  const int cap = D.cap;
+#pragma unroll 1
+  for (int half = 0; half < 2; half++) {
+    const int k0 = (c0 + half) * P.celly + r0;
+    const int jb = D.cstart[k0], je = D.cstart[k0 + 2];
+#pragma unroll 2
+    for (int j = jb; j < je; j++) {
+      if (j == i) continue;
+      const double xj = D.x[j], yj = D.y[j];
+      const double dx = xi - xj, dy = yi - yj;
+      const double d2 = (dx * dx) + (dy * dy);
+      if (d2 > P.lim2) continue;
+      const double r = sqrt(d2);
+      if (!(r < P.two_h)) continue;
+      const double k1 = d_wabd(dx, r, P), k2 = d_wabd(dy, r, P);
+      if (cnt < NBR_CAP) D.nbr[cnt * cap + i] = j;
+      cnt++;
+      const double mj = D.m[j], rj = D.rho[j];
+      const double r1 = (mj * (xj - xi)) / rj;
+      const double r3 = (mj * (yj - yi)) / rj;
+      m0 = m0 + k1 * r1;
+      m1 = m1 + k2 * r1;
+      m2 = m2 + k1 * r3;
+      m3 = m3 + k2 * r3;
+    }
+  }
+  if (cnt > NBR_CAP) {
+    atomicOr(&D.sc->err, ERR_NBR_CAP);
+    cnt = NBR_CAP;
+  }
+  D.ncnt[i] = cnt;
+  D.xnb[i] = xi;
+  D.ynb[i] = yi;
+  uint8_t fl = D.flags[i] & (uint8_t)~F_FREE;
+  double gv = 0.0;
+  if (D.tid[i] == TID_FLUID) {
+    D.vic[i] = 0;
+    gv = m0 + m3;
+    // invertmat2D, functions.f90:62-96 (1e-5 is a real4 literal)
+    const double det = m0 * m3 - m1 * m2;
+    double c1, c2, c3, c4;
+    if ((fabs(det) >= (double)1e-5f) && !isnan(det) && isfinite(det)) {
+      c1 = m3 / det; c2 = (-m1) / det; c3 = (-m2) / det; c4 = m0 / det;
+    } else {
+      c1 = 1.0; c2 = 0.0; c3 = 0.0; c4 = 1.0;
+    }
+    D.cf1[i] = c1; D.cf2[i] = c2; D.cf3[i] = c3; D.cf4[i] = c4;
+    if ((gv <= P.fs_thresh) && !(fl & F_BUFFER)) fl |= F_FREE;
+  }
+  D.gradvx[i] = gv;
+  D.flags[i] = fl;
+}
+
+// ---------------------------------------------------------------------
+// effpor + porsearch, porous.f90:32-224
+// ---------------------------------------------------------------------
+__global__ void k_effpor(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  const DevParams& P = D.P;
+  D.resist[i] = 0.0;
+  if (D.tid[i] != TID_FLUID) {
+    D.por[i] = 1.0;  // non-fluid porosity keeps its default (particle.f90:62-64)
+    return;
+  }
+  int ci, cj;
+  double xl, xr, yb, yt;
+  cell_bounds(D.cell[i], P, ci, cj, xl, xr, yb, yt);
+  const double x = D.x[i], y = D.y[i];
+  const int q = quadrant(x, y, xl, xr, yb, yt);
+  // self first, then the quadrant's other three cells (porous.f90:138-176)
+  int di[4] = {0, 0, 0, 0}, dj[4] = {0, 0, 0, 0};
+  if (q == 0) { dj[1] = -1; di[2] = 1; di[3] = 1; dj[3] = -1; }
+  else if (q == 1) { dj[1] = 1; di[2] = 1; di[3] = 1; dj[3] = 1; }
+  else if (q == 2) { di[1] = -1; dj[1] = 1; di[2] = -1; dj[3] = 1; }
+  else { di[1] = -1; di[2] = -1; dj[2] = -1; dj[3] = -1; }
+  const double por = P.por;
+  double sf = 0.0;
+  for (int cb = 0; cb < 4; cb++) {
+    const int k = (cj + dj[cb]) * P.celly + (ci + di[cb]);
+    const int qe = D.sstart[k + 1];
+    for (int s = D.sstart[k]; s < qe; s++) {
+      const double dx = x - D.sx[s], dy = y - D.sy[s];
+      const double r = sqrt((dx * dx) + (dy * dy));
+      if (r < P.two_h) sf = sf + ((((1.0 - por * 1.0) * d_wab(r, P)) * D.sm[s]) / D.srho[s]);
+    }
+  }
+  if (sf > (1.0 - por)) sf = 1.0 - por;
+  const double nn = 1.0 - sf;
+  D.por[i] = nn;
+  D.rho[i] = nn * D.rho0[i];
+  const double n3 = (nn * nn) * nn;
+  const double fch = 1.750 / sqrt(150 * n3);
+  const double ui = D.u[i], vi = D.v[i];
+  const double sp = sqrt((ui * ui) + (vi * vi));
+  const double a = -((((d_vis(D.rho0[i]) / D.rho0[i]) * P.lamk) * ((1.0 - nn) * (1.0 - nn))) /
+                     (n3 * (P.d50 * P.d50)));
+  const double b = (((fch * sp) * sqrt(P.lamk)) * (1.0 - nn)) / (sqrt(n3) * P.d50);
+  D.resist[i] = a - b;
+}
+
+// ---------------------------------------------------------------------
+// eddyvis, turbulence.f90:12-110
+// ---------------------------------------------------------------------
+__global__ void k_eddyvis(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  if (D.tid[i] != TID_FLUID) {
+    D.nut[i] = 0.0;
+    D.tke[i] = 0.0;
+    return;
+  }
+  const DevParams& P = D.P;
+  const int cap = D.cap, cnt = D.ncnt[i];
+  const double c1 = D.cf1[i], c2 = D.cf2[i], c3 = D.cf3[i], c4 = D.cf4[i];
+  const double xi = D.xnb[i], yi = D.ynb[i];
+  const double ui = D.u[i] / D.por[i], vi = D.v[i] / D.por[i];
+  double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = D.nbr[k * cap + i];
+    const Klt z = d_klt(xi - D.xnb[j], yi - D.ynb[j], P);
+    const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+    const double nj = D.por[j], mj = D.m[j], rj = D.rho[j];
+    const double du = (D.u[j] / nj) - ui, dv = (D.v[j] / nj) - vi;
+    t1 = t1 + ((mj * du) * gx) / rj;
+    t2 = t2 + ((mj * dv) * gy) / rj;
+    t3 = t3 + ((0.50 * mj) * ((du * gy) + (dv * gx))) / rj;
+  }
+  const double t4 = sqrt((2 * ((t1 * t1) + (t2 * t2))) + ((2 * t3) * (2 * t3)));
+  const double dl = P.dl;
+  D.nut[i] = ((0.150 * dl) * (0.150 * dl)) * t4;
+  D.tke[i] = (0.080 / 1.0) * ((dl * t4) * (dl * t4));
+}
+
+// ---------------------------------------------------------------------
+// int_vel, integrator.f90:459-703 (pass 1: varts of free particles)
+// ---------------------------------------------------------------------
+__global__ void k_int_vel_varts(Dev D) {
+  PIDX();
+  if (i >= n || !(D.flags[i] & F_FREE)) return;
+  const DevParams& P = D.P;
+  const int cap = D.cap, cnt = D.ncnt[i];
+  const double c1 = D.cf1[i], c2 = D.cf2[i], c3 = D.cf3[i], c4 = D.cf4[i];
+  const double xi = D.xnb[i], yi = D.ynb[i];
+  const double ui = D.u[i] / D.por[i], vi = D.v[i] / D.por[i];
+  double t1 = 0.0, t2 = 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = D.nbr[k * cap + i];
+    const Klt z = d_klt(xi - D.xnb[j], yi - D.ynb[j], P);
+    const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+    const double nj = D.por[j], vj = D.m[j] / D.rho[j];
+    t1 = t1 + (((D.u[j] / nj) - ui) * gx) * vj;
+    t2 = t2 + (((D.v[j] / nj) - vi) * gy) * vj;
+  }
+  const double uu = D.u[i], vv = D.v[i];
+  D.varts[i] = (((1.0 * (P.dl * P.dl)) * sqrt((t1 * t1) + (t2 * t2))) * sqrt((uu * uu) + (vv * vv))) /
+               ((D.sc->umax + (double)1e-6f) * D.por[i]);
+}
+
+// pass 2: intermediate velocity (viscous + SPS + tke terms, Darcy-Forchheimer)
+__global__ void __launch_bounds__(256) k_int_vel(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  const DevParams& P = D.P;
+  const double dt = D.sc->dt;
+  const bool buf = (D.flags[i] & F_BUFFER) != 0;
+  const double ni = D.por[i];
+  if (D.tid[i] == TID_FLUID && !buf) {
+    const double eps6 = (double)1e-6f;
+    const int cap = D.cap, cnt = D.ncnt[i];
+    const double c1 = D.cf1[i], c2 = D.cf2[i], c3 = D.cf3[i], c4 = D.cf4[i];
+    const double xnbi = D.xnb[i], ynbi = D.ynb[i];
+    const double xi = D.x[i], yi = D.y[i], ui = D.u[i], vi = D.v[i];
+    const double r0i = D.rho0[i], visi = d_vis(r0i);
+    const bool freei = (D.flags[i] & F_FREE) != 0;
+    const double vartsi = freei ? D.varts[i] : 0.0;
+    const double nuti = D.nut[i], tkei = D.tke[i];
+    const double cmu_wall = visi * (1.0 / r0i);
+    double us = 0.0, vs = 0.0;
+    for (int k = 0; k < cnt; k++) {
+      const int j = D.nbr[k * cap + i];
+      const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
+      const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+      const int tj = D.tid[j];
+      double cmu;
+      if (tj == TID_FLUID) {
+        const double r0j = D.rho0[j], visj = d_vis(r0j);
+        cmu = (((2 * visi) * visj) * ((1.0 / r0i) * (1.0 / r0j))) / ((visi / r0i) + (visj / r0j));
+      } else {
+        cmu = cmu_wall;
+      }
+      double vart;
+      if (freei) vart = vartsi;
+      else if ((D.flags[j] & F_FREE) && !buf) vart = (d_wab(z.r, P) * D.varts[j]) / P.wo;
+      else vart = 0.0;
+      const double pvol = D.m[j] / D.rho[j];
+      const double pd = 1.0 / ((z.r * z.r) + P.lam);
+      const double dxij = xi - D.x[j], dyij = yi - D.y[j];
+      const double uj = D.u[j], vj = D.v[j];
+      double a1 = ((((2 * pvol) * (ui - uj)) * dxij) * gx) * pd;
+      double a2 = ((((2 * pvol) * (ui - uj)) * dyij) * gy) * pd;
+      us = us + (a1 + a2) * (cmu + vart);
+      double nutj = 0.0;
+      if (tj > 2) {
+        nutj = D.nut[j];
+        if ((nuti < eps6) && (nutj < eps6)) us = us + ((a1 + a2) * (nuti + nutj)) * 0.50;
+        else us = us + (a1 + a2) * (((2 * nuti) * nutj) / (nuti + nutj));
+      }
+      a1 = ((((2 * pvol) * (vi - vj)) * dxij) * gx) * pd;
+      a2 = ((((2 * pvol) * (vi - vj)) * dyij) * gy) * pd;
+      vs = vs + (a1 + a2) * (cmu + vart);
+      if (tj > 2) {
+        if ((nuti < eps6) && (nutj < eps6)) vs = vs + ((a1 + a2) * (nuti + nutj)) * 0.50;
+        else vs = vs + (a1 + a2) * (((2 * nuti) * nutj) / (nuti + nutj));
+      }
+      const double dk = pvol * (D.tke[j] - tkei);
+      double tk = dk * gx;
+      us = us - ((tk * ni) * 2.0) / 3;
+      tk = dk * gy;
+      vs = vs - ((tk * ni) * 2.0) / 3;
+    }
+    const double den = 1.0 - ((dt * D.resist[i]) * ni);
+    us = (ui + (dt * us)) / den;
+    vs = (vi + (dt * vs)) / den;
+    vs = vs + ((P.g * ni) * dt);
+    D.us[i] = us;
+    D.vs[i] = vs;
+  }
+  if (buf) {
+    D.us[i] = D.u[i];
+    D.vs[i] = (P.g * ni) * dt;
+  }
+}
+
+// ---------------------------------------------------------------------
+// PPE assembly, isph.f90:115-456, into canonical ELL rows:
+// columns ascending, the diagonal before the slot-1 value a ghost row keeps
+// from the row initialisation (isph.f90:127), explicit zeros dropped.
+// Raw values are written first, then divided by the diagonal (Jacobi,
+// isph.f90:432-440) once the diagonal is known.
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ppe_assemble(Dev D, int want32) {
+  PIDX();
+  if (i >= n) return;
+  const DevParams& P = D.P;
+  const int cap = D.cap, cnt = D.ncnt[i];
+  const double dt = D.sc->dt;
+  const int tidi = D.tid[i];
+  const bool freei = (D.flags[i] & F_FREE) != 0;
+  const double ni = D.por[i], rhoi = D.rho[i], mi = D.m[i];
+  const double rho_i = rhoi * (1.0 / ni);
+  const double xi = D.x[i], yi = D.y[i], usi = D.us[i], vsi = D.vs[i];
+  const double xnbi = D.xnb[i], ynbi = D.ynb[i];
+  const double c1 = D.cf1[i], c2 = D.cf2[i], c3 = D.cf3[i], c4 = D.cf4[i];
+  const double rhoref = P.rho * P.rel_den;
+  double diag = 0.0, rhs = 0.0;
+  int L = 0, dpos = 0;
+  bool before = true, first_written = false;
+  for (int k = 0; k < cnt; k++) {
+    const int j = D.nbr[k * cap + i];
+    if (before && j > i) {
+      // reserve the diagonal slot (and the ghost-row slot-1 value)
+      before = false;
+      dpos = L;
+      L += 1 + ((tidi == TID_GHOST && k > 0 && !first_written) ? 1 : 0);
+    }
+    const int tj = D.tid[j];
+    const double nj = D.por[j], rhoj = D.rho[j];
+    const double rho_j = rhoj * (1.0 / nj);
+    const double pvol = D.m[j] * (1.0 / rhoj);
+    const double xj = D.x[j], yj = D.y[j];
+    const double ddx = xj - xi, ddy = yj - yi;
+    const double dd = sqrt((ddx * ddx) + (ddy * ddy));
+    const double pd = 1.0 / ((dd * dd) + P.lam);
+    const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
+    const double dx = xi - xj, dy = yi - yj;
+    bool wr = false;
+    double val = 0.0, t1, t2;
+    if (tidi == TID_FLUID) {
+      const double lamp = (freei && tj == TID_FLUID) ? (1.0 + D.rho0[j] / rhoref) : 1.0;
+      const double rho_ij = (((2 * rhoj) * rhoi) * (1.0 / (nj * ni))) *
+                            (1.0 / ((rhoj * (1.0 / nj)) + (rhoi * (1.0 / ni))));
+      const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+      t1 = (((2 * pvol) * gx) * dx) * pd;
+      t2 = (((2 * pvol) * gy) * dy) * pd;
+      val = -(t1 + t2);
+      wr = true;
+      diag = diag + (t1 + t2) * lamp;
+      if (tj == TID_FLUID) {
+        t1 = (pvol * (((D.us[j] * rho_ij) / nj) - ((usi * rho_ij) / ni))) * gx;
+        t2 = (pvol * (((D.vs[j] * rho_ij) / nj) - ((vsi * rho_ij) / ni))) * gy;
+      } else {
+        t1 = (pvol * (((D.us[j] * rho_i) / ni) - ((usi * rho_i) / ni))) * gx;
+        t2 = (pvol * (((D.vs[j] * rho_i) / ni) - ((vsi * rho_i) / ni))) * gy;
+      }
+      rhs = rhs + (t1 + t2) / dt;
+    } else if (tidi == TID_GHOST) {
+      if (tj <= 2) {
+        t1 = (((2 * pvol) * z.k1) * dx) * pd;
+        t2 = (((2 * pvol) * z.k2) * dy) * pd;
+        val = -(t1 + t2);
+        wr = true;
+        diag = diag + (t1 + t2);
+      }
+    } else if (tidi == TID_FIXED) {
+      if (tj == TID_FLUID) {
+        const double gxj = D.cf1[j] * z.k1 + D.cf2[j] * z.k2;
+        const double gyj = D.cf3[j] * z.k1 + D.cf4[j] * z.k2;
+        t1 = ((((2 * mi) * gxj) * dx) * pd) / rhoi;
+        t2 = ((((2 * mi) * gyj) * dy) * pd) / rhoi;
+        val = -(t1 + t2);
+        wr = true;
+        diag = diag + (t1 + t2);
+        t1 = ((mi * (((D.us[j] * rho_j) / nj) - ((usi * rho_j) / ni))) * gxj) / rhoi;
+        t2 = ((mi * (((D.vs[j] * rho_j) / nj) - ((vsi * rho_j) / ni))) * gyj) / rhoi;
+        rhs = rhs + (t1 + t2) / dt;
+      } else if (tj <= 2) {
+        t1 = (((2 * pvol) * z.k1) * dx) * pd;
+        t2 = (((2 * pvol) * z.k2) * dy) * pd;
+        val = -(t1 + t2);
+        wr = true;
+        diag = diag + (t1 + t2);
+      }
+    }
+    if (k == 0) first_written = wr;
+    if (wr) {
+      D.acol[L * cap + i] = j;
+      D.aval[L * cap + i] = val;
+      L++;
+    }
+  }
+  if (before) {
+    dpos = L;
+    L += 1 + ((tidi == TID_GHOST && cnt > 0 && !first_written) ? 1 : 0);
+  }
+  const double t1 = diag;
+  if (!(isfinite(t1) && t1 != 0.0)) atomicOr(&D.sc->err, ERR_DIAG);
+  D.acol[dpos * cap + i] = i;
+  D.aval[dpos * cap + i] = diag / t1;
+  const bool quirk = (tidi == TID_GHOST && cnt > 0 && D.nbr[i] < i && !first_written);
+  if (quirk) {
+    D.acol[(dpos + 1) * cap + i] = i;
+    D.aval[(dpos + 1) * cap + i] = 1.0 / t1;
+  }
+  for (int k = 0; k < L; k++) {
+    if (k == dpos || (quirk && k == dpos + 1)) continue;
+    D.aval[k * cap + i] = D.aval[k * cap + i] / t1;
+  }
+  if (want32) {
+    for (int k = 0; k < L; k++) D.aval32[k * cap + i] = (float)D.aval[k * cap + i];
+  }
+  D.alen[i] = L;
+  D.b[i] = rhs / t1;
+}
+
+// ---------------------------------------------------------------------
+// comp_vel, integrator.f90:705-843 (with the running maxima of timestep)
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* a, double v) {
+  if (v == v) atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void __launch_bounds__(256) k_comp_vel(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  const DevParams& P = D.P;
+  const double dt = D.sc->dt;
+  const bool buf = (D.flags[i] & F_BUFFER) != 0;
+  if (D.tid[i] == TID_FLUID && !buf) {
+    const int cap = D.cap, cnt = D.ncnt[i];
+    const double c1 = D.cf1[i], c2 = D.cf2[i], c3 = D.cf3[i], c4 = D.cf4[i];
+    const double xnbi = D.xnb[i], ynbi = D.ynb[i];
+    const double ni = D.por[i], rhoi = D.rho[i], pi = D.p[i];
+    double u = 0.0, v = 0.0;
+    for (int k = 0; k < cnt; k++) {
+      const int j = D.nbr[k * cap + i];
+      const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
+      const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+      const double nj = D.por[j], rhoj = D.rho[j], pj = D.p[j];
+      const double V = D.m[j] / rhoj;
+      double t1;
+      if (D.tid[j] == TID_FLUID) {
+        const double a = ((pj * rhoi) / ni) + ((pi * rhoj) / nj);
+        const double den = rhoi * ((rhoi / ni) + (rhoj / nj));
+        t1 = (2 * (((V * (ni * ni)) * a) * gx)) / den;
+        u = u - t1 * dt;
+        t1 = (2 * (((V * (ni * ni)) * a) * gy)) / den;
+        v = v - t1 * dt;
+      } else {
+        t1 = (((V * (ni * ni)) * (pj + pi)) * gx) / rhoi;
+        u = u - t1 * dt;
+        t1 = (((V * (ni * ni)) * (pj + pi)) * gy) / rhoi;
+        v = v - t1 * dt;
+      }
+    }
+    u = D.us[i] + u;
+    v = D.vs[i] + v;
+    D.u[i] = u;
+    D.v[i] = v;
+    const double vel = sqrt((u * u) + (v * v)) / ni;
+    D.vel[i] = vel;
+    atomic_max_pos(&D.sc->umax_bits, fabs(vel));
+  }
+  if (buf) {
+    D.u[i] = D.us[i];
+    D.v[i] = 0.0;
+  }
+  atomic_max_pos(&D.sc->numax_bits, fabs(D.nut[i]));
+}
+
+// comp_pos, integrator.f90:845-883
+__global__ void k_comp_pos(Dev D) {
+  PIDX();
+  if (i >= n || D.tid[i] != TID_FLUID) return;
+  const double dt = D.sc->dt;
+  const double x = ((D.x[i] + D.xs[i]) + ((dt * D.u[i]) / D.por[i])) / 2.0;
+  const double y = ((D.y[i] + D.ys[i]) + ((dt * D.v[i]) / D.por[i])) / 2.0;
+  D.x[i] = x;
+  D.y[i] = y;
+  if ((D.flags[i] & F_BUFFER) && (x > D.P.buf_xmin) && (x < D.P.buf_xmax))
+    D.flags[i] = D.flags[i] & (uint8_t)~F_BUFFER;
+}
+
+// freesurf, isph.f90:49-86: surface particles mark their neighbours
+// (plain byte stores of the same value; the order of the stores is irrelevant)
+__global__ void k_freesurf(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  if (D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER) && D.gradvx[i] <= D.P.fs_thresh) {
+    const int cap = D.cap, cnt = D.ncnt[i];
+    for (int k = 0; k < cnt; k++) D.vic[D.nbr[k * cap + i]] = 1;
+  }
+}
+
+// ---------------------------------------------------------------------
+// opt2_shift, part_shift.f90:12-191.  The surface normal and pint are zero
+// at the point of use (part_shift.f90:67-69), so the shift sums reduce to
+// -csh*h^2*frac*gradW*V; the stored shift is the capped one, the velocity
+// correction uses the uncapped shift.
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_opt2_shift(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  if (!(D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER))) return;
+  const int cnt = D.ncnt[i];
+  if (cnt == 0) return;
+  const DevParams& P = D.P;
+  const int cap = D.cap;
+  const double c1 = D.cf1[i], c2 = D.cf2[i], c3 = D.cf3[i], c4 = D.cf4[i];
+  const double xnbi = D.xnb[i], ynbi = D.ynb[i];
+  const double frac = (D.flags[i] & F_FREE) ? 0.01 : 1.0;
+  const double coef = (P.csh * (P.h * P.h)) * frac;
+  const double ui = D.u[i], vi = D.v[i];
+  double xsa = 0.0, ysa = 0.0, a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = D.nbr[k * cap + i];
+    const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
+    const double V = D.m[j] / D.rho[j];
+    xsa = xsa - ((coef * z.k1) * V);
+    ysa = ysa - ((coef * z.k2) * V);
+    const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+    const double du = D.u[j] - ui, dv = D.v[j] - vi;
+    a1 = a1 + (gx * du) * V;
+    a2 = a2 + (gy * du) * V;
+    b1 = b1 + (gx * dv) * V;
+    b2 = b2 + (gy * dv) * V;
+  }
+  const double capv = P.maxshift * P.dl;
+  const double ax = fabs(xsa), ay = fabs(ysa);
+  D.xs[i] = (xsa == 0.0) ? 0.0 : (((capv < ax) ? capv : ax) * xsa) / ax;
+  D.ys[i] = (ysa == 0.0) ? 0.0 : (((capv < ay) ? capv : ay) * ysa) / ay;
+  D.us[i] = a1 * xsa + a2 * ysa;
+  D.vs[i] = b1 * xsa + b2 * ysa;
+}
+
+__global__ void k_shift_apply(Dev D) {
+  PIDX();
+  if (i >= n) return;
+  if (D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER)) {
+    D.x[i] = D.x[i] + D.xs[i];
+    D.y[i] = D.y[i] + D.ys[i];
+    D.u[i] = D.u[i] + D.us[i];
+    D.v[i] = D.v[i] + D.vs[i];
+  }
+}
+
+// densityupdate, scalar.f90:399-426
+__global__ void k_densityupdate(Dev D) {
+  PIDX();
+  if (i >= n || D.tid[i] != TID_FLUID) return;
+  const double rhomax = D.P.rho * D.P.rel_den, rhomin = D.P.rhomin;
+  const double r0 = (0.5 * (rhomax + rhomin)) + (D.c[i] * (rhomax - rhomin));
+  D.rho0[i] = r0;
+  D.m[i] = r0 * D.vol0[i];
+  D.rho[i] = r0 * D.por[i];
+}
+
+// timestep, integrator.f90:885-953 (one thread)
+__device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
+__global__ void k_timestep(Dev D) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const DevParams& P = D.P;
+  DevScalars* S = D.sc;
+  S->umax = __longlong_as_double((long long)S->umax_bits);
+  S->numax = __longlong_as_double((long long)S->numax_bits);
+  const double dl = P.dl, sig1 = P.sig1;
+  if (!P.fixed_dt) {
+    const double t = S->t;
+    double dtn;
+    if (t <= 1.0) {
+      dtn = dmin2(dmin2(sig1 * dl / S->umax, (sig1 * (dl * dl)) / ((P.mu / P.rho) + S->numax)), 0.0010);
+    } else if ((t > 1.0) && (t <= 2.50)) {
+      float a = (float)(sig1 * dl / S->umax), b = (float)(((sig1 * (dl * dl)) * P.rho) / P.mu);
+      float mn = a;
+      if (b < mn) mn = b;
+      if (0.002f < mn) mn = 0.002f;
+      dtn = (double)mn;
+    } else if ((t > 3.0) && (t <= 4.0)) {
+      float a = (float)(sig1 * dl / S->umax), b = (float)(((sig1 * (dl * dl)) * P.rho) / P.mu);
+      float mn = a;
+      if (b < mn) mn = b;
+      if (0.004f < mn) mn = 0.004f;
+      dtn = (double)mn;
+    } else {
+      dtn = dmin2(dmin2(sig1 * dl / S->umax, (sig1 * (dl * dl)) / ((P.mu / P.rho) + S->numax)), 0.0050);
+    }
+    S->dt = dtn;
+  }
+  if (!isfinite(S->numax) || isnan(S->numax)) S->numax = 0.0;
+  double dtsol = (sig1 * (dl * dl)) / ((double)1e-9f + S->numax / P.tschmidt);
+  int solsteps = 1;
+  if (dtsol < S->dt) {
+    solsteps = (int)ceil(S->dt / dtsol);
+    dtsol = S->dt / solsteps;
+  }
+  S->dtsol = (dtsol < S->dt) ? dtsol : S->dt;
+  S->solsteps = solsteps;
+}
+
+// ---------------------------------------------------------------------
+// scalart_anisotropic / scalarupdate, scalar.f90:242-397.  The neighbour
+// list (and its r) is from the last neighbour search; positions are current.
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_scalart(Dev D) {
+  PIDX();
+  if (i >= n || D.tid[i] != TID_FLUID) return;
+  const DevParams& P = D.P;
+  const double al = 0.0025, at = 0.00025, Dm = (double)1e-9f, thr = 0.7;
+  const double eps6 = (double)1e-6f;
+  const double dt = D.sc->dt;
+  const double capv = (0.12 * (P.dl * P.dl)) / dt;
+  const int cap = D.cap, cnt = D.ncnt[i];
+  const double xnbi = D.xnb[i], ynbi = D.ynb[i], xi = D.x[i], yi = D.y[i];
+  const double ui = D.u[i], vi_ = D.v[i], ni = D.por[i], ci1 = D.c[i];
+  const double nuti = D.nut[i], vol0i = D.vol0[i];
+  const double vi = sqrt((ui * ui) + (vi_ * vi_)) / ni;
+  double cd = 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = D.nbr[k * cap + i];
+    if (D.tid[j] != TID_FLUID) continue;
+    const double ex = xnbi - D.xnb[j], ey = ynbi - D.ynb[j];
+    const double r = sqrt((ex * ex) + (ey * ey));
+    const double uj = D.u[j], vj_ = D.v[j], nj = D.por[j], c2 = D.c[j];
+    const double dx = xi - D.x[j], dy = yi - D.y[j];
+    double t1x;
+    if ((ni < thr) && (nj < thr)) {
+      const double vj = sqrt((uj * uj) + (vj_ * vj_)) / nj;
+      const double Dxxi = ui / ((sqrt((al * vi) + Dm) * vi) * ni);
+      const double Dxyi = vi_ / ((sqrt((al * vi) + Dm) * vi) * ni);
+      const double Dyxi = (-vi_) / ((sqrt((at * vi) + Dm) * vi) * ni);
+      const double Dyyi = ui / ((sqrt((at * vi) + Dm) * vi) * ni);
+      const double Dxxj = uj / ((sqrt((al * vj) + Dm) * vj) * nj);
+      const double Dxyj = vj_ / ((sqrt((al * vj) + Dm) * vj) * nj);
+      const double Dyxj = (-vj_) / ((sqrt((at * vj) + Dm) * vj) * nj);
+      const double Dyyj = uj / ((sqrt((at * vj) + Dm) * vj) * nj);
+      const double pd = 1.0 / r;
+      const double A = ((((Dxxi + Dxxj) * dx) * pd) * 0.5) + ((((Dxyi + Dxyj) * dy) * pd) * 0.5);
+      const double B = ((((Dyxi + Dyxj) * dx) * pd) * 0.5) + ((((Dyyi + Dyyj) * dy) * pd) * 0.5);
+      t1x = (A * A) + (B * B);
+      t1x = 1.0 / t1x;
+      t1x = (t1x <= capv) ? t1x : capv;
+    } else {
+      const double nutj = D.nut[j];
+      const double Dxxi = Dm + nuti / P.tschmidt;
+      const double Dxxj = Dm + nutj / P.tschmidt;
+      if ((nuti < eps6) && (nutj < eps6)) t1x = (Dxxi + Dxxj) * 0.50;
+      else t1x = ((2 * Dxxi) * Dxxj) / (Dxxi + Dxxj);
+    }
+    const double t3 = (vol0i * (ni + nj)) / (ni * nj);
+    const double hwx = d_wabd(dx, r, P), hwy = d_wabd(dy, r, P);
+    const double den = (r * r) + P.lam;
+    cd = (cd + (((((((t1x * 1.0) * t3) * (ci1 - c2)) * dx) * hwx) / den))) +
+         (((((((t1x * 1.0) * t3) * (ci1 - c2)) * dy) * hwy) / den));
+  }
+  D.cdiff[i] = cd;
+}
+
+// stime < 0: use the sub-step length dtsol from the last timestep
+__global__ void k_scalarupdate(Dev D, double stime) {
+  PIDX();
+  if (i >= n || D.tid[i] != TID_FLUID) return;
+  const double st = (stime < 0.0) ? D.sc->dtsol : stime;
+  D.c[i] = D.c[i] + D.cdiff[i] * st;
+  if ((D.flags[i] & F_BUFFER) && (D.x[i] < 1.6)) D.c[i] = 0.5;
+}

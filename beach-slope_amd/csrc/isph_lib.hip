@@ -1,0 +1,982 @@
+// isph_lib.hip -- C ABI of the MI355X ISPH per-step pipeline (declared in
+// include/isph_abi.h).  One context = one scenario, device resident, one HIP
+// stream.  Every entry point maps to one reference subroutine of the
+// production loop (main.f90:199-261) or to a fused sequence of them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/isph_abi.h"
+#include "isph_device.h"
+#include "isph_kernels.h"
+#include "isph_solver.h"
+
+static_assert(NBR_CAP == ISPH_NBR_CAP, "neighbour capacity mismatch");
+
+struct isph_ctx {
+  isph_params P;
+  Dev D;
+  hipStream_t stream = nullptr;
+  int64_t cap = 0, scap = 0;
+  int n_hi = 0;          // host-side upper bound of the live particle count
+  int ncell = 0;
+  int device = 0;
+  bool assembled = false, have_nbr = false;
+  isph_solve_stats last{};
+  std::string err;
+  std::vector<void*> allocs;
+  double* d_bn2 = nullptr;
+};
+
+#define HIPCHK(ctx, expr)                                                     \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      set_err(ctx, "%s failed: %s", #expr, hipGetErrorString(e_));            \
+      return ISPH_EHIP;                                                       \
+    }                                                                         \
+  } while (0)
+
+static void set_err(isph_ctx* c, const char* fmt, ...) {
+  if (!c) return;
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  c->err = buf;
+}
+
+static int grid_of(int n) { return std::max(1, (n + 255) / 256); }
+
+template <typename T>
+static int dalloc(isph_ctx* c, T** p, int64_t count) {
+  void* q = nullptr;
+  size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(count, 1);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    set_err(c, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return ISPH_EHIP;
+  }
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+// wab on the host, identical expression to the device (kernel.f90:34-92)
+static double host_wo(double h, double k1, double k2) {
+  return ((0.1 * 4.0) * k1) + (((1.0 - 0.1) * 6.0) * k2);
+}
+
+static void fill_devparams(isph_ctx* c) {
+  const isph_params& P = c->P;
+  DevParams& d = c->D.P;
+  d.cellx = P.cellx;
+  d.celly = P.celly;
+  d.h = P.h; d.dl = P.dl; d.lam = P.lam; d.rho = P.rho; d.rel_den = P.rel_den;
+  d.rhomin = P.rhomin; d.g = P.g; d.mu = P.mu; d.por = P.por; d.d50 = P.d50;
+  d.lamk = P.lamk; d.csh = P.csh; d.maxshift = P.maxshift; d.fs_thresh = P.fs_thresh;
+  d.sig1 = P.sig1; d.tschmidt = P.tschmidt; d.xlcutoff = P.xlcutoff; d.ytcutoff = P.ytcutoff;
+  d.buf_xmin = P.buf_xmin; d.buf_xmax = P.buf_xmax; d.tol = P.tol; d.inner_tol = P.inner_tol;
+  d.fixed_dt = P.fixed_dt;
+  d.max_iter = P.max_iter;
+  d.cs = 4.0 * P.h;
+  d.k1 = 10.0 / ((double)(28.0f * 3.14f) * (P.h * P.h));
+  d.k2 = 1.0 / ((double)(3.0f * 3.14f) * (P.h * P.h));
+  d.two_h = 2 * P.h;
+  d.lim2 = (d.two_h * d.two_h) * (1.0 + 1e-9);
+  d.wo = host_wo(P.h, d.k1, d.k2);
+}
+
+extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int32_t cellx,
+                                        int32_t celly, int32_t precision) {
+  const int64_t ncell = (int64_t)cellx * celly;
+  int64_t per = 0;
+  per += 2 * (14 * 8 + 3 + 4);                  // gathered state, ping-pong
+  per += 14 * 8;                                // derived properties
+  per += 4 + 4 * NBR_CAP;                       // neighbour list
+  per += 4 * 3;                                 // bin, rank, scan scratch
+  per += 4 + ROW_CAP * (4 + 8) + 2 * 8;         // PPE rows, rhs, solution
+  per += 6 * 8;                                 // FP64 solver vectors
+  if (precision) per += ROW_CAP * 4 + 7 * 4;    // FP32 matrix and vectors
+  int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
+  b += 8 * 2 * (n_cap / 256 + 2);               // reduction partials
+  return b + (1 << 20);
+}
+
+extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_solid_cap,
+                           isph_ctx** out) {
+  // slot-major ELL indices k*cap + i are int32: cap*ROW_CAP must stay below 2^31
+  if (!params || !out || n_cap <= 0 || n_cap * ROW_CAP >= (1ll << 31) || n_solid_cap < 0)
+    return ISPH_EARG;
+  if (params->cellx < 3 || params->celly < 3 || !(params->h > 0)) return ISPH_EARG;
+  if ((int64_t)params->cellx * params->celly * NCLS + 1 > (1ll << 30)) return ISPH_EARG;
+  isph_ctx* c = new isph_ctx();
+  c->P = *params;
+  c->cap = n_cap;
+  c->scap = std::max<int64_t>(n_solid_cap, 1);
+  c->ncell = params->cellx * params->celly;
+  hipGetDevice(&c->device);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ISPH_EHIP;
+  }
+  Dev& D = c->D;
+  D.cap = (int)n_cap;
+  fill_devparams(c);
+  int rc = 0;
+  const int64_t N = n_cap;
+  double** dd[] = {&D.x, &D.y, &D.u, &D.v, &D.us, &D.vs, &D.xs, &D.ys, &D.p, &D.m, &D.rho,
+                   &D.rho0, &D.vol0, &D.c, &D.x2, &D.y2, &D.u2, &D.v2, &D.us2, &D.vs2,
+                   &D.xs2, &D.ys2, &D.p2, &D.m2, &D.rho2, &D.rho02, &D.vol02, &D.c2,
+                   &D.por, &D.resist, &D.gradvx, &D.cf1, &D.cf2, &D.cf3, &D.cf4, &D.nut,
+                   &D.tke, &D.varts, &D.cdiff, &D.vel, &D.xnb, &D.ynb, &D.b, &D.sol,
+                   &D.r, &D.rh, &D.pp, &D.t1, &D.sv, &D.t2};
+  for (auto p : dd) rc |= dalloc(c, p, N);
+  uint8_t** bb[] = {&D.tid, &D.flags, &D.vic, &D.tid2, &D.flags2, &D.vic2};
+  for (auto p : bb) rc |= dalloc(c, p, N);
+  rc |= dalloc(c, &D.cell, N);
+  rc |= dalloc(c, &D.cell2, N);
+  rc |= dalloc(c, &D.ncnt, N);
+  rc |= dalloc(c, &D.nbr, N * NBR_CAP);
+  rc |= dalloc(c, &D.pbin, N);
+  rc |= dalloc(c, &D.prank, N);
+  rc |= dalloc(c, &D.alen, N);
+  rc |= dalloc(c, &D.acol, N * ROW_CAP);
+  rc |= dalloc(c, &D.aval, N * ROW_CAP);
+  const int64_t nbins = (int64_t)c->ncell * NCLS + 1;
+  rc |= dalloc(c, &D.cstart, c->ncell + 1);
+  rc |= dalloc(c, &D.bincnt, nbins);
+  rc |= dalloc(c, &D.binstart, nbins);
+  rc |= dalloc(c, &D.scan_tmp, nbins / SCAN_TILE + 2);
+  rc |= dalloc(c, &D.sx, c->scap);
+  rc |= dalloc(c, &D.sy, c->scap);
+  rc |= dalloc(c, &D.sm, c->scap);
+  rc |= dalloc(c, &D.srho, c->scap);
+  rc |= dalloc(c, &D.sstart, c->ncell + 1);
+  rc |= dalloc(c, &D.part, 2 * (N / 256 + 2));
+  rc |= dalloc(c, &D.sc, 1);
+  rc |= dalloc(c, &c->d_bn2, 1);
+  if (params->precision) {
+    rc |= dalloc(c, &D.aval32, N * ROW_CAP);
+    float** ff[] = {&D.r32, &D.rh32, &D.pp32, &D.t132, &D.sv32, &D.t232, &D.d32};
+    for (auto p : ff) rc |= dalloc(c, p, N);
+  }
+  if (rc) {
+    std::string e = c->err;
+    isph_destroy(c);
+    return ISPH_EHIP;
+  }
+  DevScalars s{};
+  s.dt = params->dt;
+  s.dtsol = params->dt;
+  s.solsteps = 1;
+  if (hipMemcpy(D.sc, &s, sizeof s, hipMemcpyHostToDevice) != hipSuccess) {
+    isph_destroy(c);
+    return ISPH_EHIP;
+  }
+  hipMemset(D.sstart, 0, sizeof(int32_t) * (c->ncell + 1));
+  hipMemset(D.vic, 0, N);
+  *out = c;
+  return 0;
+}
+
+extern "C" int isph_destroy(isph_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" const char* isph_last_error(const isph_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+extern "C" int isph_set_params(isph_ctx* c, const isph_params* p) {
+  if (!c || !p) return ISPH_EARG;
+  if (p->cellx != c->P.cellx || p->celly != c->P.celly) return ISPH_EARG;
+  if (p->precision && !c->D.aval32) {
+    set_err(c, "context was created without the FP32 buffers");
+    return ISPH_EARG;
+  }
+  c->P = *p;
+  fill_devparams(c);
+  return 0;
+}
+
+// containing interior cell with half-open bounds (internal.f90:80-83)
+static int locate_cell(const isph_params& P, double x, double y) {
+  const double cs = 4.0 * P.h;
+  if (!(x == x) || !(y == y)) return -1;
+  int cj = (int)std::floor(x / cs) + 1;
+  int ci = P.celly - 2 - (int)std::floor(y / cs);
+  auto xl = [&](int j) { return (double)(j - 1) * cs; };
+  auto xr = [&](int j) { return (double)(j)*cs; };
+  auto yb = [&](int i) { return (double)(P.celly - 2 - i) * cs; };
+  auto yt = [&](int i) { return (double)(P.celly - 1 - i) * cs; };
+  for (int it = 0; it < 3; it++) {
+    if (cj < 0 || cj >= P.cellx) break;
+    if (x < xl(cj)) cj--;
+    else if (x >= xr(cj)) cj++;
+  }
+  for (int it = 0; it < 3; it++) {
+    if (ci < 0 || ci >= P.celly) break;
+    if (y < yb(ci)) ci++;
+    else if (y >= yt(ci)) ci--;
+  }
+  if (cj < 1 || cj > P.cellx - 2 || ci < 1 || ci > P.celly - 2) return -1;
+  if (!(x >= xl(cj) && x < xr(cj) && y >= yb(ci) && y < yt(ci))) return -1;
+  return cj * P.celly + ci;
+}
+
+static int sync(isph_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int read_scalars(isph_ctx* c, DevScalars* s) {
+  HIPCHK(c, hipMemcpyAsync(s, c->D.sc, sizeof *s, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int isph_upload(isph_ctx* c, int64_t n, const double* x, const double* y,
+                           const double* u, const double* v, const double* p, const double* m,
+                           const double* rho, const double* rho0, const double* vol0,
+                           const double* cc, const int32_t* tid, const uint8_t* flags) {
+  if (!c) return ISPH_EARG;
+  if (n < 0 || n > c->cap) {
+    set_err(c, "upload of %lld particles exceeds capacity %lld", (long long)n, (long long)c->cap);
+    return ISPH_ECAP;
+  }
+  if (n > 0 && (!x || !y || !u || !v || !p || !m || !rho || !rho0 || !vol0 || !cc || !tid || !flags))
+    return ISPH_EARG;
+  // initial binning: walls before fluid in each cell, input order kept
+  std::vector<int64_t> key(n);
+  for (int64_t i = 0; i < n; i++) {
+    int k = locate_cell(c->P, x[i], y[i]);
+    if (k < 0) {
+      set_err(c, "particle %lld at (%g, %g) is outside the interior cells", (long long)i, x[i], y[i]);
+      return ISPH_EARG;
+    }
+    if (tid[i] < 1 || tid[i] > 3) {
+      set_err(c, "particle %lld has tid %d", (long long)i, tid[i]);
+      return ISPH_EARG;
+    }
+    key[i] = 2 * (int64_t)k + (tid[i] == ISPH_TID_FLUID ? 1 : 0);
+  }
+  std::vector<int64_t> ord(n);
+  for (int64_t i = 0; i < n; i++) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  std::vector<double> buf(n);
+  auto put = [&](double* dst, const double* src) -> int {
+    for (int64_t k = 0; k < n; k++) buf[k] = src[ord[k]];
+    HIPCHK(c, hipMemcpy(dst, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+  };
+  Dev& D = c->D;
+  int rc = 0;
+  rc |= put(D.x, x); rc |= put(D.y, y); rc |= put(D.u, u); rc |= put(D.v, v); rc |= put(D.p, p);
+  rc |= put(D.m, m); rc |= put(D.rho, rho); rc |= put(D.rho0, rho0); rc |= put(D.vol0, vol0);
+  rc |= put(D.c, cc);
+  rc |= put(D.sol, p);  // fsol starts from the initial pressures (memory.f90:113-144)
+  if (rc) return rc;
+  std::vector<double> zeros(n, 0.0);
+  HIPCHK(c, hipMemcpy(D.us, zeros.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(D.vs, zeros.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(D.xs, zeros.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(D.ys, zeros.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  std::vector<uint8_t> tb(n), fb(n), vb(n);
+  std::vector<int32_t> cb(n);
+  for (int64_t k = 0; k < n; k++) {
+    tb[k] = (uint8_t)tid[ord[k]];
+    fb[k] = flags[ord[k]] & (ISPH_F_BUFFER | ISPH_F_FREE);
+    vb[k] = (flags[ord[k]] & ISPH_F_VICINITY) ? 1 : 0;
+    cb[k] = (int32_t)(key[ord[k]] / 2);
+  }
+  HIPCHK(c, hipMemcpy(D.tid, tb.data(), n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(D.flags, fb.data(), n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(D.vic, vb.data(), n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(D.cell, cb.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  // porosity defaults to 1 (particle.f90:62-64)
+  std::vector<double> ones(n, 1.0);
+  HIPCHK(c, hipMemcpy(D.por, ones.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  std::vector<int32_t> cs(c->ncell + 1, 0);
+  for (int64_t k = 0; k < n; k++) cs[cb[k] + 1]++;
+  for (int k = 0; k < c->ncell; k++) cs[k + 1] += cs[k];
+  HIPCHK(c, hipMemcpy(D.cstart, cs.data(), sizeof(int32_t) * (c->ncell + 1), hipMemcpyHostToDevice));
+  DevScalars s;
+  HIPCHK(c, hipMemcpy(&s, D.sc, sizeof s, hipMemcpyDeviceToHost));
+  s.n = (int)n;
+  s.err = 0;
+  s.umax_bits = 0;
+  s.numax_bits = 0;
+  HIPCHK(c, hipMemcpy(D.sc, &s, sizeof s, hipMemcpyHostToDevice));
+  c->n_hi = (int)n;
+  c->assembled = false;
+  c->have_nbr = false;
+  return 0;
+}
+
+extern "C" int isph_upload_solids(isph_ctx* c, int64_t ns, const double* x, const double* y,
+                                  const double* m, const double* rho) {
+  if (!c) return ISPH_EARG;
+  if (ns < 0 || ns > c->scap) {
+    set_err(c, "upload of %lld solids exceeds capacity %lld", (long long)ns, (long long)c->scap);
+    return ISPH_ECAP;
+  }
+  std::vector<int32_t> key(ns);
+  std::vector<int32_t> cs(c->ncell + 1, 0);
+  for (int64_t i = 0; i < ns; i++) {
+    int k = locate_cell(c->P, x[i], y[i]);
+    if (k < 0) {
+      set_err(c, "solid %lld at (%g, %g) is outside the interior cells", (long long)i, x[i], y[i]);
+      return ISPH_EARG;
+    }
+    key[i] = k;
+    cs[k + 1]++;
+  }
+  for (int k = 0; k < c->ncell; k++) cs[k + 1] += cs[k];
+  std::vector<int32_t> fill(cs.begin(), cs.end());
+  std::vector<double> bx(ns), by(ns), bm(ns), br(ns);
+  for (int64_t i = 0; i < ns; i++) {
+    int64_t d = fill[key[i]]++;
+    bx[d] = x[i]; by[d] = y[i]; bm[d] = m[i]; br[d] = rho[i];
+  }
+  Dev& D = c->D;
+  if (ns) {
+    HIPCHK(c, hipMemcpy(D.sx, bx.data(), 8 * ns, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(D.sy, by.data(), 8 * ns, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(D.sm, bm.data(), 8 * ns, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(D.srho, br.data(), 8 * ns, hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemcpy(D.sstart, cs.data(), 4 * (c->ncell + 1), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int isph_download(isph_ctx* c, int64_t* n_out, double* x, double* y, double* u,
+                             double* v, double* p, double* m, double* rho, double* rho0,
+                             double* vol0, double* cc, int32_t* tid, uint8_t* flags) {
+  if (!c) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  const int64_t n = s.n;
+  c->n_hi = s.n;
+  if (n_out) *n_out = n;
+  Dev& D = c->D;
+  struct { double* dst; double* src; } f[] = {{x, D.x}, {y, D.y}, {u, D.u}, {v, D.v}, {p, D.p},
+                                              {m, D.m}, {rho, D.rho}, {rho0, D.rho0},
+                                              {vol0, D.vol0}, {cc, D.c}};
+  for (auto& e : f)
+    if (e.dst) HIPCHK(c, hipMemcpy(e.dst, e.src, 8 * n, hipMemcpyDeviceToHost));
+  if (tid) {
+    std::vector<uint8_t> t(n);
+    HIPCHK(c, hipMemcpy(t.data(), D.tid, n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) tid[i] = t[i];
+  }
+  if (flags) {
+    std::vector<uint8_t> fl(n), vi(n);
+    HIPCHK(c, hipMemcpy(fl.data(), D.flags, n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(vi.data(), D.vic, n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) flags[i] = fl[i] | (vi[i] ? ISPH_F_VICINITY : 0);
+  }
+  return 0;
+}
+
+extern "C" int isph_get_scalars(isph_ctx* c, isph_scalars* o) {
+  if (!c || !o) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  o->t = s.t; o->dt = s.dt; o->dtsol = s.dtsol; o->umax = s.umax; o->numax = s.numax;
+  o->solsteps = s.solsteps; o->n = s.n;
+  return 0;
+}
+
+extern "C" int isph_set_scalars(isph_ctx* c, const isph_scalars* in) {
+  if (!c || !in) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  s.t = in->t; s.dt = in->dt; s.dtsol = in->dtsol; s.umax = in->umax; s.numax = in->numax;
+  s.solsteps = in->solsteps;
+  HIPCHK(c, hipMemcpy(c->D.sc, &s, sizeof s, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------
+#define LAUNCH(kern, nthreads, ...)                                                  \
+  do {                                                                               \
+    hipLaunchKernelGGL(kern, dim3(grid_of(nthreads)), dim3(256), 0, c->stream, __VA_ARGS__); \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess) {                                                          \
+      set_err(c, "launch of %s failed: %s", #kern, hipGetErrorString(e_));          \
+      return ISPH_EHIP;                                                              \
+    }                                                                                \
+  } while (0)
+
+static int check_err_flags(isph_ctx* c) {
+  int e = 0;
+  HIPCHK(c, hipMemcpyAsync(&e, &c->D.sc->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (e) {
+    int z = 0;
+    HIPCHK(c, hipMemcpy(&c->D.sc->err, &z, sizeof z, hipMemcpyHostToDevice));
+    if (e & ERR_NBR_CAP) {
+      set_err(c, "a particle has more than %d neighbours", NBR_CAP);
+      return ISPH_ECAP;
+    }
+    if (e & ERR_DIAG) {
+      set_err(c, "zero or non-finite PPE diagonal (isolated particle)");
+      return ISPH_ENONFINITE;
+    }
+  }
+  return 0;
+}
+
+extern "C" int isph_projection(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_projection, c->n_hi, c->D);
+  return 0;
+}
+
+static void swap_state(Dev& D) {
+  std::swap(D.x, D.x2); std::swap(D.y, D.y2); std::swap(D.u, D.u2); std::swap(D.v, D.v2);
+  std::swap(D.us, D.us2); std::swap(D.vs, D.vs2); std::swap(D.xs, D.xs2); std::swap(D.ys, D.ys2);
+  std::swap(D.p, D.p2); std::swap(D.m, D.m2); std::swap(D.rho, D.rho2); std::swap(D.rho0, D.rho02);
+  std::swap(D.vol0, D.vol02); std::swap(D.c, D.c2); std::swap(D.tid, D.tid2);
+  std::swap(D.flags, D.flags2); std::swap(D.vic, D.vic2); std::swap(D.cell, D.cell2);
+}
+
+static int scan_bins(isph_ctx* c, int len) {
+  const int ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, c->stream, c->D.bincnt,
+                     c->D.binstart, c->D.scan_tmp, len);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, c->stream, c->D.scan_tmp, ntiles,
+                     (int32_t*)nullptr);
+  hipLaunchKernelGGL(k_scan_add, dim3(grid_of(len)), dim3(256), 0, c->stream, c->D.binstart,
+                     c->D.scan_tmp, len);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+static int rebin(isph_ctx* c, int alt) {
+  const int nb = c->ncell * NCLS + 1;
+  LAUNCH(k_fill_i32, nb, c->D.bincnt, nb, 0);
+  LAUNCH(k_rebin_classify, c->n_hi, c->D, alt);
+  LAUNCH(k_rebin_rank, c->n_hi, c->D);
+  int rc = scan_bins(c, nb);
+  if (rc) return rc;
+  LAUNCH(k_rebin_scatter, c->n_hi, c->D);
+  swap_state(c->D);
+  LAUNCH(k_rebin_finish, c->ncell + 1, c->D, c->ncell);
+  c->assembled = false;
+  c->have_nbr = false;
+  return 0;
+}
+
+extern "C" int isph_cellshift(isph_ctx* c) { return c ? rebin(c, 0) : ISPH_EARG; }
+extern "C" int isph_cellshiftalt(isph_ctx* c) { return c ? rebin(c, 1) : ISPH_EARG; }
+extern "C" int isph_resetid(isph_ctx* c) {
+  // matid is the sorted index: nothing to do on the device (isph.f90:88-113)
+  return c ? 0 : ISPH_EARG;
+}
+
+static int neighbour_nocheck(isph_ctx* c) {
+  LAUNCH(k_neighbour, c->n_hi, c->D);
+  c->have_nbr = true;
+  return 0;
+}
+
+extern "C" int isph_neighbour(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  int rc = neighbour_nocheck(c);
+  if (rc) return rc;
+  return check_err_flags(c);
+}
+
+extern "C" int isph_effpor(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_effpor, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_eddyvis(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  if (!c->have_nbr) { set_err(c, "eddyvis before neighbour"); return ISPH_ESTATE; }
+  LAUNCH(k_eddyvis, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_int_vel(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  if (!c->have_nbr) { set_err(c, "int_vel before neighbour"); return ISPH_ESTATE; }
+  LAUNCH(k_int_vel_varts, c->n_hi, c->D);
+  LAUNCH(k_int_vel, c->n_hi, c->D);
+  return 0;
+}
+
+static int assemble_nocheck(isph_ctx* c) {
+  LAUNCH(k_ppe_assemble, c->n_hi, c->D, c->P.precision ? 1 : 0);
+  c->assembled = true;
+  return 0;
+}
+
+extern "C" int isph_ppe_assemble(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  if (!c->have_nbr) { set_err(c, "ppe assembly before neighbour"); return ISPH_ESTATE; }
+  int rc = assemble_nocheck(c);
+  if (rc) return rc;
+  return check_err_flags(c);
+}
+
+// run BiCGStab chunks until the device reports done
+static int bicg_loop(isph_ctx* c, bool inner, int* iters_out, int* brk_out, int* conv_out,
+                     double* rel_out) {
+  const int n = c->n_hi;
+  int chunk = 4;
+  DevScalars s;
+  for (int it = 0;; ) {
+    for (int k = 0; k < chunk; k++) {
+      if (inner) {
+        LAUNCH(k_in_1, n, c->D);
+        LAUNCH(k_in_2, n, c->D);
+        LAUNCH(k_in_3, n, c->D);
+        LAUNCH(k_in_4, n, c->D);
+      } else {
+        LAUNCH(k_bicg_1, n, c->D);
+        LAUNCH(k_bicg_2, n, c->D);
+        LAUNCH(k_bicg_3, n, c->D);
+        LAUNCH(k_bicg_4, n, c->D);
+      }
+    }
+    it += chunk;
+    int rc = read_scalars(c, &s);
+    if (rc) return rc;
+    if (s.done) break;
+    if (chunk < 32) chunk *= 2;
+    if (it > c->P.max_iter + 64) break;
+  }
+  *iters_out = s.iters;
+  *brk_out = s.brk;
+  *conv_out = s.conv;
+  *rel_out = s.rel;
+  return 0;
+}
+
+static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
+  if (!c->assembled) {
+    set_err(c, "ppe solve before assembly");
+    return ISPH_ESTATE;
+  }
+  const int n = c->n_hi;
+  isph_solve_stats st{};
+  // ||b||
+  LAUNCH(k_dot, n, c->D, c->D.b, c->D.b, c->d_bn2);
+  double bn2 = 0;
+  HIPCHK(c, hipMemcpyAsync(&bn2, c->d_bn2, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double bnorm = std::sqrt(bn2);
+  st.bnorm = bnorm;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  s.bnorm = bnorm;
+  HIPCHK(c, hipMemcpy(&c->D.sc->bnorm, &bnorm, 8, hipMemcpyHostToDevice));
+  if (bnorm == 0.0 || !std::isfinite(bnorm)) {
+    if (bnorm == 0.0) {
+      HIPCHK(c, hipMemsetAsync(c->D.sol, 0, 8 * (size_t)c->cap, c->stream));
+      st.converged = 1;
+    }
+  } else if (c->P.precision == ISPH_PPE_FP64) {
+    LAUNCH(k_bicg_init, n, c->D);
+    rc = read_scalars(c, &s);
+    if (rc) return rc;
+    int it = 0, brk = 0, conv = s.conv;
+    double rel = s.rel;
+    if (!s.done) {
+      rc = bicg_loop(c, false, &it, &brk, &conv, &rel);
+      if (rc) return rc;
+    }
+    st.iterations = it;
+    st.breakdown = brk;
+    st.converged = conv;
+    st.rel_residual = rel;
+  } else {
+    int total = 0;
+    for (int outer = 1; outer <= c->P.max_outer; outer++) {
+      LAUNCH(k_ref_resid, n, c->D);
+      rc = read_scalars(c, &s);
+      if (rc) return rc;
+      const double rn = std::sqrt(s.rr);
+      st.rel_residual = rn / bnorm;
+      st.outer = outer - 1;
+      if (rn <= c->P.tol * bnorm) {
+        st.converged = 1;
+        break;
+      }
+      LAUNCH(k_in_init, n, c->D);
+      int it = 0, brk = 0, conv = 0;
+      double rel = 0;
+      rc = read_scalars(c, &s);
+      if (rc) return rc;
+      if (!s.done) {
+        rc = bicg_loop(c, true, &it, &brk, &conv, &rel);
+        if (rc) return rc;
+      }
+      total += it;
+      if (brk) st.breakdown = 1;
+      LAUNCH(k_ref_update, n, c->D);
+      if (outer == c->P.max_outer) {
+        LAUNCH(k_ref_resid, n, c->D);
+        rc = read_scalars(c, &s);
+        if (rc) return rc;
+        st.rel_residual = std::sqrt(s.rr) / bnorm;
+        st.outer = outer;
+        st.converged = std::sqrt(s.rr) <= c->P.tol * bnorm;
+      }
+    }
+    st.iterations = total;
+  }
+  LAUNCH(k_copy_sol_to_p, n, c->D);
+  c->last = st;
+  if (stats) *stats = st;
+  if (!std::isfinite(st.rel_residual) || !std::isfinite(bnorm)) {
+    set_err(c, "non-finite PPE residual");
+    return ISPH_ENONFINITE;
+  }
+  if (!st.converged) {
+    set_err(c, "PPE solve did not converge: %d iterations, rel residual %g", st.iterations,
+            st.rel_residual);
+    return ISPH_ENOCONV;
+  }
+  return 0;
+}
+
+extern "C" int isph_ppe_solve(isph_ctx* c, isph_solve_stats* stats) {
+  if (!c) return ISPH_EARG;
+  return ppe_solve_impl(c, stats);
+}
+
+extern "C" int isph_ppesolve(isph_ctx* c, isph_solve_stats* stats) {
+  int rc = isph_ppe_assemble(c);
+  if (rc) return rc;
+  return isph_ppe_solve(c, stats);
+}
+
+extern "C" int isph_comp_vel(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_comp_vel, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_comp_pos(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_comp_pos, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_freesurf(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_freesurf, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_opt2_shift(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_opt2_shift, c->n_hi, c->D);
+  LAUNCH(k_shift_apply, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_densityupdate(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_densityupdate, c->n_hi, c->D);
+  return 0;
+}
+
+__global__ void k_reset_max(Dev D) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    D.sc->umax_bits = 0ull;
+    D.sc->numax_bits = 0ull;
+  }
+}
+
+extern "C" int isph_timestep(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  hipLaunchKernelGGL(k_timestep, dim3(1), dim3(64), 0, c->stream, c->D);
+  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(64), 0, c->stream, c->D);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+extern "C" int isph_scalart_anisotropic(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  if (!c->have_nbr) { set_err(c, "scalar transport before neighbour"); return ISPH_ESTATE; }
+  LAUNCH(k_scalart, c->n_hi, c->D);
+  return 0;
+}
+
+extern "C" int isph_scalarupdate(isph_ctx* c, double stime) {
+  if (!c || !(stime >= 0.0)) return ISPH_EARG;
+  LAUNCH(k_scalarupdate, c->n_hi, c->D, stime);
+  return 0;
+}
+
+__global__ void k_numax_init(Dev D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) D.sc->umax_bits = 0ull;
+  if (i < D.sc->n) atomic_max_pos(&D.sc->numax_bits, fabs(D.nut[i]));
+}
+
+extern "C" int isph_init_state(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  int rc = isph_neighbour(c);
+  if (rc) return rc;
+  LAUNCH(k_effpor, c->n_hi, c->D);
+  LAUNCH(k_eddyvis, c->n_hi, c->D);
+  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(64), 0, c->stream, c->D);
+  LAUNCH(k_numax_init, c->n_hi, c->D);
+  rc = isph_timestep(c);
+  if (rc) return rc;
+  return sync(c);
+}
+
+__global__ void k_advance_t(Dev D) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->t = D.sc->t + D.sc->dt;
+}
+
+// one production-loop iteration, main.f90:199-261
+static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
+  DevScalars s;
+  int rc = read_scalars(c, &s);  // solsteps from the previous timestep
+  if (rc) return rc;
+  c->n_hi = s.n;
+  if (s.solsteps < 1 || s.solsteps > ISPH_MAX_SOLSTEPS) {
+    set_err(c, "scalar sub-step count %d outside [1, %d]", s.solsteps, ISPH_MAX_SOLSTEPS);
+    return ISPH_ENONFINITE;
+  }
+  hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, c->stream, c->D);
+  for (int q = 0; q < s.solsteps; q++) {
+    LAUNCH(k_scalart, c->n_hi, c->D);
+    LAUNCH(k_scalarupdate, c->n_hi, c->D, -1.0);
+  }
+  LAUNCH(k_projection, c->n_hi, c->D);
+  rc = rebin(c, 0);
+  if (rc) return rc;
+  rc = neighbour_nocheck(c);
+  if (rc) return rc;
+  LAUNCH(k_effpor, c->n_hi, c->D);
+  LAUNCH(k_eddyvis, c->n_hi, c->D);
+  LAUNCH(k_int_vel_varts, c->n_hi, c->D);
+  LAUNCH(k_int_vel, c->n_hi, c->D);
+  rc = assemble_nocheck(c);
+  if (rc) return rc;
+  rc = check_err_flags(c);
+  if (rc) return rc;
+  rc = ppe_solve_impl(c, stats);
+  if (rc == ISPH_ENONFINITE || rc == ISPH_EHIP || rc == ISPH_ESTATE) return rc;
+  *solve_rc = rc;
+  LAUNCH(k_comp_vel, c->n_hi, c->D);
+  LAUNCH(k_comp_pos, c->n_hi, c->D);
+  rc = rebin(c, 1);
+  if (rc) return rc;
+  rc = neighbour_nocheck(c);
+  if (rc) return rc;
+  LAUNCH(k_freesurf, c->n_hi, c->D);
+  LAUNCH(k_effpor, c->n_hi, c->D);
+  LAUNCH(k_opt2_shift, c->n_hi, c->D);
+  LAUNCH(k_shift_apply, c->n_hi, c->D);
+  LAUNCH(k_densityupdate, c->n_hi, c->D);
+  rc = isph_timestep(c);
+  if (rc) return rc;
+  LAUNCH(k_eddyvis, c->n_hi, c->D);
+  return check_err_flags(c);
+}
+
+extern "C" int isph_step(isph_ctx* c, int32_t nsteps, isph_solve_stats* stats) {
+  if (!c || nsteps < 0) return ISPH_EARG;
+  int worst = 0;
+  for (int k = 0; k < nsteps; k++) {
+    int srcode = 0;
+    int rc = step_once(c, stats, &srcode);
+    if (rc) return rc;
+    if (srcode) worst = srcode;
+  }
+  int rc = sync(c);
+  if (rc) return rc;
+  return worst;
+}
+
+// ---------------------------------------------------------------------
+// dumps for phase-by-phase parity
+// ---------------------------------------------------------------------
+extern "C" int isph_dump_cells(isph_ctx* c, int32_t* cell_of, int32_t* cell_start) {
+  if (!c) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  if (cell_of) HIPCHK(c, hipMemcpy(cell_of, c->D.cell, 4 * (size_t)s.n, hipMemcpyDeviceToHost));
+  if (cell_start)
+    HIPCHK(c, hipMemcpy(cell_start, c->D.cstart, 4 * (size_t)(c->ncell + 1), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+template <typename T>
+static int dump_ell(isph_ctx* c, const T* src, T* dst, int width, int n) {
+  std::vector<T> tmp((size_t)width * c->cap);
+  HIPCHK(c, hipMemcpy(tmp.data(), src, sizeof(T) * (size_t)width * c->cap, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < width; k++) dst[(size_t)i * width + k] = tmp[(size_t)k * c->cap + i];
+  return 0;
+}
+
+extern "C" int isph_dump_neighbours(isph_ctx* c, int32_t* count, int32_t* list) {
+  if (!c) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  if (count) HIPCHK(c, hipMemcpy(count, c->D.ncnt, 4 * (size_t)s.n, hipMemcpyDeviceToHost));
+  if (list) return dump_ell(c, c->D.nbr, list, NBR_CAP, s.n);
+  return 0;
+}
+
+extern "C" int isph_dump_props(isph_ctx* c, double* porosity, double* resist, double* gradvx,
+                               double* coff, double* nut, double* tke, double* varts,
+                               double* cdiff, double* vel, double* us, double* vs, double* xs,
+                               double* ys) {
+  if (!c) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  const size_t n = s.n;
+  Dev& D = c->D;
+  struct { double* dst; double* src; } f[] = {{porosity, D.por}, {resist, D.resist},
+                                              {gradvx, D.gradvx}, {nut, D.nut}, {tke, D.tke},
+                                              {varts, D.varts}, {cdiff, D.cdiff}, {vel, D.vel},
+                                              {us, D.us}, {vs, D.vs}, {xs, D.xs}, {ys, D.ys}};
+  for (auto& e : f)
+    if (e.dst) HIPCHK(c, hipMemcpy(e.dst, e.src, 8 * n, hipMemcpyDeviceToHost));
+  if (coff) {
+    std::vector<double> a(n), b(n), cc(n), d(n);
+    HIPCHK(c, hipMemcpy(a.data(), D.cf1, 8 * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(b.data(), D.cf2, 8 * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cc.data(), D.cf3, 8 * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(d.data(), D.cf4, 8 * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+      coff[4 * i] = a[i]; coff[4 * i + 1] = b[i]; coff[4 * i + 2] = cc[i]; coff[4 * i + 3] = d[i];
+    }
+  }
+  return 0;
+}
+
+extern "C" int isph_dump_ppe(isph_ctx* c, int32_t* row_len, int32_t* col, double* val,
+                             double* rhs) {
+  if (!c) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  const int n = s.n;
+  if (row_len) HIPCHK(c, hipMemcpy(row_len, c->D.alen, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  if (rhs) HIPCHK(c, hipMemcpy(rhs, c->D.b, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  if (col && (rc = dump_ell(c, c->D.acol, col, ROW_CAP, n))) return rc;
+  if (val && (rc = dump_ell(c, c->D.aval, val, ROW_CAP, n))) return rc;
+  return 0;
+}
+
+extern "C" int isph_ppe_set_rhs(isph_ctx* c, const double* rhs, const double* x0) {
+  if (!c) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  if (rhs) HIPCHK(c, hipMemcpy(c->D.b, rhs, 8 * (size_t)s.n, hipMemcpyHostToDevice));
+  if (x0) HIPCHK(c, hipMemcpy(c->D.sol, x0, 8 * (size_t)s.n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int isph_ppe_get_solution(isph_ctx* c, double* x) {
+  if (!c || !x) return ISPH_EARG;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(x, c->D.sol, 8 * (size_t)s.n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Load an explicit system in canonical row form (row-major, ISPH_NBR_CAP+2 wide)
+// and mark it assembled: used by the solver known-answer tests.
+extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, const int32_t* col,
+                                const double* val, const double* rhs, const double* x0) {
+  if (!c || n < 0 || n > c->cap) return ISPH_EARG;
+  std::vector<int32_t> L(n), C((size_t)ROW_CAP * c->cap, 0);
+  std::vector<double> V((size_t)ROW_CAP * c->cap, 0.0);
+  std::vector<float> V32((size_t)ROW_CAP * c->cap, 0.0f);
+  for (int i = 0; i < n; i++) {
+    if (len[i] < 0 || len[i] > ROW_CAP) return ISPH_EARG;
+    L[i] = len[i];
+    for (int k = 0; k < len[i]; k++) {
+      const int j = col[(size_t)i * ROW_CAP + k];
+      if (j < 0 || j >= n) return ISPH_EARG;
+      C[(size_t)k * c->cap + i] = j;
+      V[(size_t)k * c->cap + i] = val[(size_t)i * ROW_CAP + k];
+      V32[(size_t)k * c->cap + i] = (float)val[(size_t)i * ROW_CAP + k];
+    }
+  }
+  HIPCHK(c, hipMemcpy(c->D.alen, L.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->D.acol, C.data(), 4 * C.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->D.aval, V.data(), 8 * V.size(), hipMemcpyHostToDevice));
+  if (c->D.aval32)
+    HIPCHK(c, hipMemcpy(c->D.aval32, V32.data(), 4 * V32.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->D.b, rhs, 8 * (size_t)n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->D.sol, x0, 8 * (size_t)n, hipMemcpyHostToDevice));
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  s.n = n;
+  HIPCHK(c, hipMemcpy(c->D.sc, &s, sizeof s, hipMemcpyHostToDevice));
+  c->n_hi = n;
+  c->assembled = true;
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// batched form: independent scenarios advanced together
+// ---------------------------------------------------------------------
+struct isph_batch {
+  std::vector<isph_ctx*> ctx;
+};
+
+extern "C" int isph_batch_create(int32_t n_scen, isph_ctx* const* ctxs, isph_batch** out) {
+  if (n_scen <= 0 || !ctxs || !out) return ISPH_EARG;
+  isph_batch* b = new isph_batch();
+  b->ctx.assign(ctxs, ctxs + n_scen);
+  *out = b;
+  return 0;
+}
+
+extern "C" int isph_batch_step(isph_batch* b, int32_t nsteps) {
+  if (!b) return ISPH_EARG;
+  int worst = 0;
+  for (int k = 0; k < nsteps; k++) {
+    for (isph_ctx* c : b->ctx) {
+      int srcode = 0;
+      int rc = step_once(c, nullptr, &srcode);
+      if (rc) return rc;
+      if (srcode) worst = srcode;
+    }
+  }
+  for (isph_ctx* c : b->ctx) {
+    int rc = sync(c);
+    if (rc) return rc;
+  }
+  return worst;
+}
+
+extern "C" int isph_batch_destroy(isph_batch* b) {
+  delete b;
+  return 0;
+}

@@ -1,0 +1,371 @@
+// isph_solver.h -- PPE solve kernels: BiCGStab in the update order of the
+// reference's bicgstab (solver.f90:182-213) with y = A x in the row order of
+// spmv2 (solver.f90:84-127), plus the FP32-inner / FP64-refinement form.
+//
+// Each BiCGStab iteration is four launches.  Dot products are reduced in a
+// fixed blocked tree (block_tree / final_tree) and the scalars alpha, omega,
+// beta are formed on the device by the last block to finish, so the host
+// only polls the `done` flag every few iterations.
+#pragma once
+#include "isph_device.h"
+
+// y_i = sum_k A_ik x_col(k), sequential in row order (spmv2)
+template <typename T, typename V>
+__device__ __forceinline__ V row_spmv(const Dev& D, const T* __restrict__ val, int i,
+                                      const V* __restrict__ x) {
+  const int cap = D.cap, L = D.alen[i];
+  V acc = (V)0;
+  for (int k = 0; k < L; k++) acc = acc + (V)val[k * cap + i] * x[D.acol[k * cap + i]];
+  return acc;
+}
+
+// ---- FP64 path ----
+// r = b - A x ; rh = r ; p = r ; partial (r.r) ; last block: rho = rr
+__global__ void __launch_bounds__(256) k_bicg_init(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double prod = 0.0;
+  if (i < n) {
+    const double ax = row_spmv<double, double>(D, D.aval, i, D.sol);
+    const double r = D.b[i] - ax;
+    D.r[i] = r;
+    D.rh[i] = r;
+    D.pp[i] = r;
+    prod = r * r;
+  }
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (last_block(&D.sc->counter[0])) {
+    const double rr = final_tree(D.part, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      DevScalars* S = D.sc;
+      S->rho = rr;
+      S->rr = rr;
+      S->iters = 0;
+      S->brk = 0;
+      S->conv = 0;
+      S->done = 0;
+      S->rel = sqrt(rr) / S->bnorm;
+      if (sqrt(rr) <= D.P.tol * S->bnorm) {
+        S->conv = 1;
+        S->done = 1;
+      }
+      S->counter[0] = 0;
+    }
+  }
+}
+
+// t1 = A p ; alpha = rho / (t1 . rh)
+__global__ void __launch_bounds__(256) k_bicg_1(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double prod = 0.0;
+  if (i < n) {
+    const double t = row_spmv<double, double>(D, D.aval, i, D.pp);
+    D.t1[i] = t;
+    prod = t * D.rh[i];
+  }
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (last_block(&D.sc->counter[1])) {
+    const double d = final_tree(D.part, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      D.sc->alpha = D.sc->rho / d;
+      D.sc->counter[1] = 0;
+    }
+  }
+}
+
+// s = r - alpha t1 ; t2 = A s (s at the neighbours formed on the fly) ;
+// omega = (t2 . s) / (t2 . t2)
+__global__ void __launch_bounds__(256) k_bicg_2(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  const double alpha = D.sc->alpha;
+  double p1 = 0.0, p2 = 0.0;
+  if (i < n) {
+    const int cap = D.cap, L = D.alen[i];
+    double acc = 0.0;
+    for (int k = 0; k < L; k++) {
+      const int j = D.acol[k * cap + i];
+      const double sj = D.r[j] - alpha * D.t1[j];
+      acc = acc + D.aval[k * cap + i] * sj;
+    }
+    const double si = D.r[i] - alpha * D.t1[i];
+    D.sv[i] = si;
+    D.t2[i] = acc;
+    p1 = acc * si;
+    p2 = acc * acc;
+  }
+  const double s1 = block_tree(p1, lds);
+  const double s2 = block_tree(p2, lds);
+  if (threadIdx.x == 0) {
+    D.part[blockIdx.x] = s1;
+    D.part[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&D.sc->counter[2])) {
+    const double d1 = final_tree(D.part, gridDim.x, lds);
+    const double d2 = final_tree(D.part + gridDim.x, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      D.sc->omega = d1 / d2;
+      D.sc->counter[2] = 0;
+    }
+  }
+}
+
+// x = (x + alpha p) + omega s ; r = s - omega t2 ; rho' = r . rh ; rr = r . r
+__global__ void __launch_bounds__(256) k_bicg_3(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  const double alpha = D.sc->alpha, omega = D.sc->omega;
+  double p1 = 0.0, p2 = 0.0;
+  if (i < n) {
+    const double si = D.sv[i];
+    D.sol[i] = (D.sol[i] + alpha * D.pp[i]) + omega * si;
+    const double r = si - omega * D.t2[i];
+    D.r[i] = r;
+    p1 = r * D.rh[i];
+    p2 = r * r;
+  }
+  const double s1 = block_tree(p1, lds);
+  const double s2 = block_tree(p2, lds);
+  if (threadIdx.x == 0) {
+    D.part[blockIdx.x] = s1;
+    D.part[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&D.sc->counter[3])) {
+    const double rhon = final_tree(D.part, gridDim.x, lds);
+    const double rr = final_tree(D.part + gridDim.x, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      DevScalars* S = D.sc;
+      S->iters = S->iters + 1;
+      S->rr = rr;
+      S->rel = sqrt(rr) / S->bnorm;
+      if (sqrt(rr) <= D.P.tol * S->bnorm) {
+        S->conv = 1;
+        S->done = 1;
+      } else if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 ||
+                 !isfinite(rhon)) {
+        S->brk = 1;
+        S->done = 1;
+      } else {
+        S->beta = (rhon / S->rho) * (alpha / omega);
+        S->rho = rhon;
+        if (S->iters >= D.P.max_iter) S->done = 1;
+      }
+      S->counter[3] = 0;
+    }
+  }
+}
+
+// p = r + beta (p - omega t1)
+__global__ void k_bicg_4(Dev D) {
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.sc->n) return;
+  const double beta = D.sc->beta, omega = D.sc->omega;
+  D.pp[i] = D.r[i] + beta * (D.pp[i] - omega * D.t1[i]);
+}
+
+// generic dot (a . b) into *out through the fixed tree (used for ||b||^2)
+__global__ void __launch_bounds__(256) k_dot(Dev D, const double* a, const double* b,
+                                             double* out) {
+  __shared__ double lds[RED_BLOCK];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  const double prod = (i < n) ? a[i] * b[i] : 0.0;
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (last_block(&D.sc->counter[0])) {
+    const double d = final_tree(D.part, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      *out = d;
+      D.sc->counter[0] = 0;
+    }
+  }
+}
+
+// pressure scatter (isph.f90:468-486): the solution vector is indexed by row
+__global__ void k_copy_sol_to_p(Dev D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D.sc->n) D.p[i] = D.sol[i];
+}
+
+// ---- precision form: FP64 residual, FP32 inner BiCGStab, FP64 update ----
+// r = b - A64 x (FP64), r32 = (float) r ; rn2 = r.r (FP64)
+__global__ void __launch_bounds__(256) k_ref_resid(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double prod = 0.0;
+  if (i < n) {
+    const double ax = row_spmv<double, double>(D, D.aval, i, D.sol);
+    const double r = D.b[i] - ax;
+    D.r[i] = r;
+    prod = r * r;
+  }
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (last_block(&D.sc->counter[0])) {
+    const double rr = final_tree(D.part, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      D.sc->rr = rr;
+      D.sc->rel = sqrt(rr) / D.sc->bnorm;
+      D.sc->counter[0] = 0;
+    }
+  }
+}
+
+// inner init: d = 0, r32 = rh32 = p32 = (float) r ; rho = rr0 = r32.r32 (FP64)
+__global__ void __launch_bounds__(256) k_in_init(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double prod = 0.0;
+  if (i < n) {
+    const float r = (float)D.r[i];
+    D.d32[i] = 0.0f;
+    D.r32[i] = r;
+    D.rh32[i] = r;
+    D.pp32[i] = r;
+    prod = (double)r * (double)r;
+  }
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (last_block(&D.sc->counter[0])) {
+    const double rr = final_tree(D.part, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      DevScalars* S = D.sc;
+      S->rho = rr;
+      S->stop = D.P.inner_tol * sqrt(rr);
+      S->iters = 0;
+      S->brk = 0;
+      S->done = (rr == 0.0);
+      S->counter[0] = 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_in_1(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double prod = 0.0;
+  if (i < n) {
+    const float t = row_spmv<float, float>(D, D.aval32, i, D.pp32);
+    D.t132[i] = t;
+    prod = (double)t * (double)D.rh32[i];
+  }
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (last_block(&D.sc->counter[1])) {
+    const double d = final_tree(D.part, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      D.sc->alpha = D.sc->rho / d;
+      D.sc->counter[1] = 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_in_2(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  const float af = (float)D.sc->alpha;
+  double p1 = 0.0, p2 = 0.0;
+  if (i < n) {
+    const int cap = D.cap, L = D.alen[i];
+    float acc = 0.0f;
+    for (int k = 0; k < L; k++) {
+      const int j = D.acol[k * cap + i];
+      const float sj = D.r32[j] - af * D.t132[j];
+      acc = acc + D.aval32[k * cap + i] * sj;
+    }
+    const float si = D.r32[i] - af * D.t132[i];
+    D.sv32[i] = si;
+    D.t232[i] = acc;
+    p1 = (double)acc * (double)si;
+    p2 = (double)acc * (double)acc;
+  }
+  const double s1 = block_tree(p1, lds);
+  const double s2 = block_tree(p2, lds);
+  if (threadIdx.x == 0) {
+    D.part[blockIdx.x] = s1;
+    D.part[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&D.sc->counter[2])) {
+    const double d1 = final_tree(D.part, gridDim.x, lds);
+    const double d2 = final_tree(D.part + gridDim.x, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      D.sc->omega = d1 / d2;
+      D.sc->counter[2] = 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_in_3(Dev D) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  const double alpha = D.sc->alpha, omega = D.sc->omega;
+  const float af = (float)alpha, of = (float)omega;
+  double p1 = 0.0, p2 = 0.0;
+  if (i < n) {
+    const float si = D.sv32[i];
+    D.d32[i] = (D.d32[i] + af * D.pp32[i]) + of * si;
+    const float r = si - of * D.t232[i];
+    D.r32[i] = r;
+    p1 = (double)r * (double)D.rh32[i];
+    p2 = (double)r * (double)r;
+  }
+  const double s1 = block_tree(p1, lds);
+  const double s2 = block_tree(p2, lds);
+  if (threadIdx.x == 0) {
+    D.part[blockIdx.x] = s1;
+    D.part[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&D.sc->counter[3])) {
+    const double rhon = final_tree(D.part, gridDim.x, lds);
+    const double rr = final_tree(D.part + gridDim.x, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+      DevScalars* S = D.sc;
+      S->iters = S->iters + 1;
+      if (sqrt(rr) <= S->stop) {
+        S->done = 1;
+      } else if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 ||
+                 !isfinite(rhon)) {
+        S->brk = 1;
+        S->done = 1;
+      } else {
+        S->beta = (rhon / S->rho) * (alpha / omega);
+        S->rho = rhon;
+        if (S->iters >= D.P.max_iter) S->done = 1;
+      }
+      S->counter[3] = 0;
+    }
+  }
+}
+
+__global__ void k_in_4(Dev D) {
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.sc->n) return;
+  const float bf = (float)D.sc->beta, of = (float)D.sc->omega;
+  D.pp32[i] = D.r32[i] + bf * (D.pp32[i] - of * D.t132[i]);
+}
+
+// x += (double) d
+__global__ void k_ref_update(Dev D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D.sc->n) D.sol[i] = D.sol[i] + (double)D.d32[i];
+}

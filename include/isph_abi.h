@@ -1,0 +1,205 @@
+/*
+ * isph_abi.h -- C ABI of the MI355X-native ISPH per-step particle pipeline.
+ *
+ * The reference program drives its per-step pipeline through argument-less
+ * Fortran module subroutines that read and write module globals
+ * (main.f90:199-272; state owned by initialize.f90:21-83).  This header
+ * replaces those subroutines with one entry point each, operating on an
+ * opaque device-resident context.  A Fortran driver binds them with
+ * iso_c_binding (beach-slope_amd/fortran/isph_gpu.f90); Python binds them
+ * with ctypes (beach-slope_amd/__init__.py).
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative ISPH_E* code; the
+ *     message of the last failure is returned by isph_last_error().
+ *   - Host arrays passed in are copied, never retained.  Particle state is
+ *     device resident between calls; only downloads synchronise.
+ *   - Particles are held in cell-sorted order: the index of a particle is the
+ *     reference's PPE row / matid minus one after `resetid` (isph.f90:88-113).
+ *   - A context owns one HIP stream and is not thread safe.
+ *   - Cells are numbered (ci, cj) with ci = 0..celly-1 counted from the top
+ *     of the domain and cj = 0..cellx-1 from the left; the reference's
+ *     dpcell(i,j) is (ci, cj) = (i-1, j-1).  Cell key = cj*celly + ci, the
+ *     column-major order that resetid walks (isph.f90:97-103).
+ */
+#ifndef ISPH_ABI_H
+#define ISPH_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ISPH_OK = 0,
+  ISPH_EHIP = -1,       /* HIP runtime error                                  */
+  ISPH_EARG = -2,       /* bad argument (size, null pointer, particle outside) */
+  ISPH_ECAP = -3,       /* capacity overflow (particles, neighbours, cells)   */
+  ISPH_ENONFINITE = -4, /* zero / non-finite PPE diagonal or residual         */
+  ISPH_ENOCONV = -5,    /* PPE solver did not reach its tolerance             */
+  ISPH_ESTATE = -6      /* call out of order (e.g. solve before assembly)     */
+};
+
+/* Particle type ids (particle.f90:11): 1 fixed wall, 2 ghost wall, 3 fluid. */
+enum { ISPH_TID_FIXED = 1, ISPH_TID_GHOST = 2, ISPH_TID_FLUID = 3 };
+
+/* Flag bits carried per particle (the logicals of particle.f90:42-43). */
+enum { ISPH_F_BUFFER = 1, ISPH_F_FREE = 2, ISPH_F_VICINITY = 4 };
+
+/* Maximum neighbours per particle (the reference's per-slot capacity
+ * `fplist`, memory.f90:25).  Exceeding it is ISPH_ECAP. */
+#define ISPH_NBR_CAP 64
+
+/* Upper bound on the scalar sub-steps per step (solsteps of timestep,
+ * integrator.f90:943-949); a larger count is reported as ISPH_ENONFINITE. */
+#define ISPH_MAX_SOLSTEPS 100000
+
+/* PPE precision modes. */
+enum { ISPH_PPE_FP64 = 0, ISPH_PPE_FP32_REFINE = 1 };
+
+/*
+ * Run parameters: the module variables of initialize.f90:21-83 plus the
+ * constants hard-coded in porous.f90:111, turbulence.f90:16 and the derived
+ * values of setup.f90:10-150.  All lengths in metres, SI units.
+ */
+typedef struct isph_params {
+  int32_t cellx, celly;   /* cell grid incl. one halo ring (setup.f90:51-52)   */
+  int32_t fixed_dt;       /* 1: dt stays fixed; 0: adaptive (integrator.f90:927-937) */
+  int32_t precision;      /* ISPH_PPE_FP64 or ISPH_PPE_FP32_REFINE             */
+  int32_t max_iter;       /* BiCGStab iteration cap per solve                  */
+  int32_t max_outer;      /* refinement sweeps cap (precision form)            */
+  double h;               /* smoothing length h1 (setup.f90:47)                */
+  double dl;              /* lattice spacing dl (setup.f90:53)                 */
+  double lam;             /* 1e-3*h^2 (setup.f90:82)                           */
+  double rho;             /* reference density (initialize.f90:23)             */
+  double rel_den;         /* rhomax/rho (initialize.f90:27)                    */
+  double rhomin;          /* (initialize.f90:24)                               */
+  double g;               /* gravity, negative (initialize.f90:23)             */
+  double mu;              /* (initialize.f90:39)                               */
+  double por;             /* porosity of the porous layer (initialize.f90:26)  */
+  double d50;             /* grain size (porous.f90:111)                       */
+  double lamk;            /* Darcy constant (porous.f90:111)                   */
+  double csh;             /* shifting coefficient (initialize.f90:24)          */
+  double maxshift;        /* shift cap factor (initialize.f90:40)              */
+  double fs_thresh;       /* lamfs*maxdivr (initialize.f90:45-46)              */
+  double sig1;            /* time-step coefficient (initialize.f90:39)         */
+  double tschmidt;        /* turbulent Schmidt number (initialize.f90:25)      */
+  double xlcutoff;        /* outflow cut of cellshiftalt (integrator.f90:61)   */
+  double ytcutoff;
+  double buf_xmin;        /* buffer release window of comp_pos (integrator.f90:867-868) */
+  double buf_xmax;
+  double dt;              /* initial (or fixed) time step                      */
+  double tol;             /* PPE relative residual target ||b-Ax||/||b||       */
+  double inner_tol;       /* FP32 inner solve relative tolerance (precision form) */
+} isph_params;
+
+/* Statistics of the last PPE solve. */
+typedef struct isph_solve_stats {
+  int32_t iterations;     /* BiCGStab iterations (inner, summed over sweeps)   */
+  int32_t outer;          /* refinement sweeps (0 for the FP64 path)           */
+  int32_t converged;      /* 1 if the target was met                           */
+  int32_t breakdown;      /* 1 if rho or omega became zero / non-finite        */
+  double rel_residual;    /* final ||b-Ax||/||b|| (recursive for FP64 path)    */
+  double bnorm;           /* ||b||                                             */
+} isph_solve_stats;
+
+/* Time-stepping scalars (module variables t, dt, dtsol, solsteps, umax, numax). */
+typedef struct isph_scalars {
+  double t, dt, dtsol, umax, numax;
+  int32_t solsteps;
+  int32_t n;              /* live particle count                               */
+} isph_scalars;
+
+typedef struct isph_ctx isph_ctx;
+
+/* ---- lifecycle (replaces init/memalloc/matrixid, setup.f90:10, memory.f90:10-145) ---- */
+int isph_create(const isph_params *params, int64_t n_cap, int64_t n_solid_cap,
+                isph_ctx **out);
+int isph_destroy(isph_ctx *ctx);
+const char *isph_last_error(const isph_ctx *ctx);
+int isph_set_params(isph_ctx *ctx, const isph_params *params);
+
+/* ---- state transfer (caller-owned host SoA; copied) ----
+ * isph_upload bins the particles into cells (half-open cell bounds as in
+ * internal.f90:82), walls before fluid in each cell, input order kept
+ * (boundary.f90:159-185 then internal.f90:80-91).  flags: ISPH_F_* bits. */
+int isph_upload(isph_ctx *ctx, int64_t n, const double *x, const double *y,
+                const double *u, const double *v, const double *p,
+                const double *m, const double *rho, const double *rho0,
+                const double *vol0, const double *c, const int32_t *tid,
+                const uint8_t *flags);
+/* Static porous-solid particles (the porlist of boundary.f90:523-593). */
+int isph_upload_solids(isph_ctx *ctx, int64_t ns, const double *x,
+                       const double *y, const double *m, const double *rho);
+/* Download in the current cell-sorted order; any pointer may be NULL. */
+int isph_download(isph_ctx *ctx, int64_t *n, double *x, double *y, double *u,
+                  double *v, double *p, double *m, double *rho, double *rho0,
+                  double *vol0, double *c, int32_t *tid, uint8_t *flags);
+int isph_get_scalars(isph_ctx *ctx, isph_scalars *out);
+int isph_set_scalars(isph_ctx *ctx, const isph_scalars *in);
+
+/* ---- one entry per reference subroutine (main.f90:210-260) ---- */
+int isph_projection(isph_ctx *ctx);              /* isph.f90:17-47          */
+int isph_cellshift(isph_ctx *ctx);               /* integrator.f90:251-392  */
+int isph_resetid(isph_ctx *ctx);                 /* isph.f90:88-113         */
+int isph_neighbour(isph_ctx *ctx);               /* search.f90:13-399       */
+int isph_effpor(isph_ctx *ctx);                  /* porous.f90:32-224       */
+int isph_eddyvis(isph_ctx *ctx);                 /* turbulence.f90:12-110   */
+int isph_int_vel(isph_ctx *ctx);                 /* integrator.f90:459-703  */
+int isph_ppesolve(isph_ctx *ctx, isph_solve_stats *stats); /* isph.f90:115-488 */
+int isph_comp_vel(isph_ctx *ctx);                /* integrator.f90:705-843  */
+int isph_comp_pos(isph_ctx *ctx);                /* integrator.f90:845-883  */
+int isph_cellshiftalt(isph_ctx *ctx);            /* integrator.f90:33-249   */
+int isph_freesurf(isph_ctx *ctx);                /* isph.f90:49-86          */
+int isph_opt2_shift(isph_ctx *ctx);              /* part_shift.f90:12-191   */
+int isph_densityupdate(isph_ctx *ctx);           /* scalar.f90:399-426      */
+int isph_timestep(isph_ctx *ctx);                /* integrator.f90:885-953  */
+int isph_scalart_anisotropic(isph_ctx *ctx);     /* scalar.f90:242-362      */
+int isph_scalarupdate(isph_ctx *ctx, double stime); /* scalar.f90:364-397   */
+
+/* The two halves of ppesolve, for phase-by-phase parity:
+ * assembly (isph.f90:123-456) and solve + pressure scatter (isph.f90:462-486). */
+int isph_ppe_assemble(isph_ctx *ctx);
+int isph_ppe_solve(isph_ctx *ctx, isph_solve_stats *stats);
+
+/* ---- fused entry points ----
+ * isph_init_state: first binning + resetid + neighbour + effpor + eddyvis, the
+ * state main.f90 holds when its production loop starts.
+ * isph_step: nsteps production-loop iterations (main.f90:199-261) in the
+ * reference's order; the stats of the last solve go to *stats (may be NULL). */
+int isph_init_state(isph_ctx *ctx);
+int isph_step(isph_ctx *ctx, int32_t nsteps, isph_solve_stats *stats);
+
+/* ---- parity / debug dumps (device -> host copies of derived arrays) ----
+ * Arrays are in the current sorted order.  Any pointer may be NULL. */
+int isph_dump_cells(isph_ctx *ctx, int32_t *cell_of_particle,
+                    int32_t *cell_start /* cellx*celly+1 */);
+int isph_dump_neighbours(isph_ctx *ctx, int32_t *count /* n */,
+                         int32_t *list /* n*ISPH_NBR_CAP, row-major */);
+int isph_dump_props(isph_ctx *ctx, double *porosity, double *resist,
+                    double *gradvx, double *coff /* 4n */, double *nut,
+                    double *tke, double *varts, double *cdiff, double *vel,
+                    double *us, double *vs, double *xs, double *ys);
+/* PPE rows in canonical order: row_len[n], col/val [n*(ISPH_NBR_CAP+2)] row-major, rhs[n]. */
+int isph_dump_ppe(isph_ctx *ctx, int32_t *row_len, int32_t *col, double *val,
+                  double *rhs);
+/* Upload an external PPE right-hand side / initial guess and solve only
+ * (for solver tests on the assembled matrix). */
+int isph_ppe_set_rhs(isph_ctx *ctx, const double *rhs, const double *x0);
+int isph_ppe_get_solution(isph_ctx *ctx, double *x);
+
+/* ---- batched form: many independent scenarios in one context set ---- */
+typedef struct isph_batch isph_batch;
+int isph_batch_create(int32_t n_scen, isph_ctx *const *ctxs, isph_batch **out);
+int isph_batch_step(isph_batch *b, int32_t nsteps);
+int isph_batch_destroy(isph_batch *b);
+
+/* Device memory a context of this capacity needs, in bytes (for admission). */
+int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap,
+                             int32_t cellx, int32_t celly, int32_t precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISPH_ABI_H */
