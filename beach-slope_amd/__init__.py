@@ -61,7 +61,8 @@ class Params(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32),
-                ("breakdown", C.c_int32), ("rel_residual", C.c_double), ("bnorm", C.c_double)]
+                ("breakdown", C.c_int32), ("rel_residual", C.c_double), ("bnorm", C.c_double),
+                ("nnz", C.c_double)]
 
 
 class Scalars(C.Structure):
@@ -81,7 +82,7 @@ EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_step", "isph_dump_cells", "isph_dump_neighbours", "isph_dump_props",
            "isph_dump_ppe", "isph_ppe_set_rhs", "isph_ppe_get_solution",
            "isph_batch_create", "isph_batch_step", "isph_batch_destroy",
-           "isph_footprint_bytes") + tuple("isph_" + p for p in PHASES)
+           "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read") + tuple("isph_" + p for p in PHASES)
 
 _lib = None
 
@@ -126,6 +127,8 @@ def lib():
     L.isph_batch_create.argtypes = [i32, C.POINTER(vp), C.POINTER(vp)]
     L.isph_batch_step.argtypes = [vp, i32]
     L.isph_batch_destroy.argtypes = [vp]
+    L.isph_prof_enable.argtypes = [vp, i32]
+    L.isph_prof_read.argtypes = [vp, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
     L.isph_footprint_bytes.restype = i64
     L.isph_footprint_bytes.argtypes = [i64, i64, i32, i32, i32]
     _lib = L
@@ -153,10 +156,11 @@ class Scenario:
         self._ctx = C.c_void_p()
         self._check(L.isph_create(C.byref(self._P), self.cap, max(case.ns, 1),
                                   C.byref(self._ctx)), "isph_create", ctx=False)
-        parts = {k: np.ascontiguousarray(v) for k, v in case.parts.items()}
-        args = [_ptr(parts[k]) for k in PART_FIELDS] + [_ptr(parts["tid"].astype(np.int32)),
-                                                        _ptr(parts["flags"])]
-        self._keep = parts
+        parts = {k: np.ascontiguousarray(case.parts[k], dtype=np.float64) for k in PART_FIELDS}
+        parts["tid"] = np.ascontiguousarray(case.parts["tid"], dtype=np.int32)
+        parts["flags"] = np.ascontiguousarray(case.parts["flags"], dtype=np.uint8)
+        self._keep = parts  # the arrays must outlive the call that reads them
+        args = [_ptr(parts[k]) for k in PART_FIELDS] + [_ptr(parts["tid"]), _ptr(parts["flags"])]
         self._check(L.isph_upload(self._ctx, case.n, *args), "isph_upload")
         if case.ns:
             s = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in case.solids.items()}
@@ -301,6 +305,16 @@ class Scenario:
         x0 = np.ascontiguousarray(x0, np.float64)
         self._check(lib().isph_load_system(self._ctx, len(rhs), _ptr(length), _ptr(col),
                                            _ptr(val), _ptr(rhs), _ptr(x0)), "isph_load_system")
+
+    def prof_enable(self, on: bool = True):
+        self._check(lib().isph_prof_enable(self._ctx, int(on)), "isph_prof_enable")
+
+    def prof_read(self, kernel: str):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self._check(lib().isph_prof_read(self._ctx, kernel.encode(), C.byref(ms), C.byref(cnt)),
+                    "isph_prof_read")
+        return ms.value, cnt.value
 
     def solution(self) -> np.ndarray:
         x = np.empty(self.n)

@@ -213,6 +213,20 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
     return c
 
 
+def flume_650k(steps: int = 2000, dt: float = 5.0e-5) -> Case:
+    """The 650k-particle throughput workload: the 10 m x 0.4 m flume of
+    `porous_flume` at dx = 0.0025 (4000 x 160 fluid lattice, 3 wall layers,
+    scalar patch at the toe), with the beach solids left out.  The porous
+    beach makes the reference's projection diverge within a few steps at this
+    depth (a volume jump m/rho = V0/n at the beach surface and at the walls
+    gives comp_vel a spurious 2 p_i sum_j V_j gradW term that grows with the
+    absolute pressure); without the solids the run stays bounded."""
+    c = porous_flume(steps=steps, dt=dt, name="flume_650k")
+    c.solids = {k: v[:0] for k, v in c.solids.items()}
+    c.meta["beach_solids"] = False
+    return c
+
+
 def scenario_mix(n_scen: int = 64, seed: int = 2024, n_min: int = 20_000,
                  n_max: int = 300_000, log_uniform: bool = False, steps: int = 1000):
     """Independent porous-beach scenarios of uneven particle count with
@@ -241,6 +255,43 @@ def scenario_mix(n_scen: int = 64, seed: int = 2024, n_min: int = 20_000,
         c.params["tol"] = 1.0e-8
         cases.append(c)
     return cases
+
+
+def match_wall_volumes(c: Case) -> Case:
+    """Give each wall particle the particle volume of the porous medium it sits
+    in: density n*rho with n from the solids' kernel sum (porous.f90:52-100,
+    182-190), so m/rho = V0/n matches the neighbouring fluid.  Walls never have
+    their density updated (effpor acts on fluid only), so this persists."""
+    from scipy.spatial import cKDTree
+
+    if c.ns == 0:
+        return c
+    P = c.params
+    h, por = P["h"], P["por"]
+    wall = c.parts["tid"] != TID_FLUID
+    wx, wy = c.parts["x"][wall], c.parts["y"][wall]
+    tree = cKDTree(np.c_[c.solids["x"], c.solids["y"]])
+    pairs = tree.query_ball_point(np.c_[wx, wy], r=2 * h)
+    k1 = 10.0 / (float(np.float32(28.0) * np.float32(3.14)) * h * h)
+    k2 = 1.0 / (float(np.float32(3.0) * np.float32(3.14)) * h * h)
+    sf = np.zeros(wx.size)
+    for i, js in enumerate(pairs):
+        if not js:
+            continue
+        js = np.asarray(js)
+        r = np.hypot(wx[i] - c.solids["x"][js], wy[i] - c.solids["y"][js])
+        q = r / h
+        a = 2.0 - q
+        t1 = np.where(r < h, 4.0 - 6.0 * q * q + 3.0 * q ** 3, a ** 3)
+        t2 = np.where(r < h, q ** 3 - 6.0 * q + 6.0, a ** 3)
+        w = 0.1 * t1 * k1 + 0.9 * t2 * k2
+        sf[i] = np.sum((1.0 - por) * w * c.solids["m"][js] / c.solids["rho"][js])
+    n = 1.0 - np.minimum(sf, 1.0 - por)
+    rho = c.parts["rho"].copy()
+    rho[wall] = rho[wall] * n
+    c.parts["rho"] = rho
+    c.meta["wall_volume_matched"] = True
+    return c
 
 
 def tiny_box(nx: int = 12, ny: int = 10, dx: float = 0.01, jitter: float = 0.05,

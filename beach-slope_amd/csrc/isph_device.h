@@ -46,6 +46,7 @@ struct DevScalars {
   double rho, alpha, omega, beta, rr, bnorm, stop, rel;
   int iters, done, conv, brk;
   unsigned int counter[4];
+  unsigned long long nnz;  // stored PPE entries of the last assembly
 };
 
 // error bits in DevScalars::err

@@ -578,6 +578,7 @@ __global__ void __launch_bounds__(256) k_ppe_assemble(Dev D, int want32) {
   }
   D.alen[i] = L;
   D.b[i] = rhs / t1;
+  atomicAdd(&D.sc->nnz, (unsigned long long)L);
 }
 
 // ---------------------------------------------------------------------

@@ -19,8 +19,18 @@
 
 static_assert(NBR_CAP == ISPH_NBR_CAP, "neighbour capacity mismatch");
 
+// HIP-event timing of one kernel class
+struct KTimer {
+  const char* name;
+  double total_ms = 0;
+  int64_t launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool, pending;
+};
+
 struct isph_ctx {
   isph_params P;
+  bool prof = false;
+  KTimer timers[4] = {{"spmv_p"}, {"spmv_s"}, {"spmv_p32"}, {"spmv_s32"}};
   Dev D;
   hipStream_t stream = nullptr;
   int64_t cap = 0, scap = 0;
@@ -190,7 +200,11 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
 extern "C" int isph_destroy(isph_ctx* c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
-  for (void* p : c->allocs) hipFree(p);
+  for (void* p : c->allocs) (void)hipFree(p);
+  for (KTimer& t : c->timers) {
+    for (auto& pr : t.pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    for (auto& pr : t.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -426,6 +440,87 @@ extern "C" int isph_set_scalars(isph_ctx* c, const isph_scalars* in) {
     }                                                                                \
   } while (0)
 
+static std::pair<hipEvent_t, hipEvent_t> timer_pair(KTimer& t) {
+  if (!t.pool.empty()) {
+    auto p = t.pool.back();
+    t.pool.pop_back();
+    return p;
+  }
+  std::pair<hipEvent_t, hipEvent_t> p;
+  (void)hipEventCreate(&p.first);
+  (void)hipEventCreate(&p.second);
+  return p;
+}
+
+// collect the elapsed times of completed launches (called after a sync)
+static void timers_collect(isph_ctx* c) {
+  for (KTimer& t : c->timers) {
+    for (auto& pr : t.pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        t.total_ms += ms;
+        t.launches += 1;
+      }
+      t.pool.push_back(pr);
+    }
+    t.pending.clear();
+  }
+}
+
+#define TIMED_LAUNCH(tidx, kern, nthreads, ...)                                      \
+  do {                                                                               \
+    std::pair<hipEvent_t, hipEvent_t> ev_{};                                         \
+    if (c->prof) {                                                                   \
+      ev_ = timer_pair(c->timers[tidx]);                                             \
+      (void)hipEventRecord(ev_.first, c->stream);                                    \
+    }                                                                                \
+    LAUNCH(kern, nthreads, __VA_ARGS__);                                             \
+    if (c->prof) {                                                                   \
+      (void)hipEventRecord(ev_.second, c->stream);                                   \
+      c->timers[tidx].pending.push_back(ev_);                                        \
+    }                                                                                \
+  } while (0)
+
+extern "C" int isph_prof_enable(isph_ctx* c, int32_t on) {
+  if (!c) return ISPH_EARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  timers_collect(c);
+  c->prof = on != 0;
+  for (KTimer& t : c->timers) {
+    t.total_ms = 0;
+    t.launches = 0;
+  }
+  return 0;
+}
+
+extern "C" int isph_prof_read(isph_ctx* c, const char* kernel, double* total_ms,
+                              int64_t* launches) {
+  if (!c || !kernel) return ISPH_EARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  timers_collect(c);
+  for (KTimer& t : c->timers) {
+    if (std::strcmp(t.name, kernel) == 0) {
+      if (total_ms) *total_ms = t.total_ms;
+      if (launches) *launches = t.launches;
+      return 0;
+    }
+  }
+  return ISPH_EARG;
+}
+
+// one-block reduction of the partials written by the preceding row kernel
+#define FINALIZE_OUT(op, nrows, outp)                                                \
+  do {                                                                               \
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->D, (op),     \
+                       grid_of(nrows), (outp));                                      \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess) {                                                          \
+      set_err(c, "launch of k_finalize failed: %s", hipGetErrorString(e_));         \
+      return ISPH_EHIP;                                                              \
+    }                                                                                \
+  } while (0)
+#define FINALIZE(op, nrows) FINALIZE_OUT(op, nrows, (double*)nullptr)
+
 static int check_err_flags(isph_ctx* c) {
   int e = 0;
   HIPCHK(c, hipMemcpyAsync(&e, &c->D.sc->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -525,7 +620,12 @@ extern "C" int isph_int_vel(isph_ctx* c) {
   return 0;
 }
 
+__global__ void k_zero_nnz(Dev D) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->nnz = 0;
+}
+
 static int assemble_nocheck(isph_ctx* c) {
+  hipLaunchKernelGGL(k_zero_nnz, dim3(1), dim3(64), 0, c->stream, c->D);
   LAUNCH(k_ppe_assemble, c->n_hi, c->D, c->P.precision ? 1 : 0);
   c->assembled = true;
   return 0;
@@ -548,20 +648,27 @@ static int bicg_loop(isph_ctx* c, bool inner, int* iters_out, int* brk_out, int*
   for (int it = 0;; ) {
     for (int k = 0; k < chunk; k++) {
       if (inner) {
-        LAUNCH(k_in_1, n, c->D);
-        LAUNCH(k_in_2, n, c->D);
+        TIMED_LAUNCH(2, k_in_1, n, c->D);
+        FINALIZE(FIN_IN_ALPHA, n);
+        TIMED_LAUNCH(3, k_in_2, n, c->D);
+        FINALIZE(FIN_IN_OMEGA, n);
         LAUNCH(k_in_3, n, c->D);
+        FINALIZE(FIN_IN_BETA, n);
         LAUNCH(k_in_4, n, c->D);
       } else {
-        LAUNCH(k_bicg_1, n, c->D);
-        LAUNCH(k_bicg_2, n, c->D);
+        TIMED_LAUNCH(0, k_bicg_1, n, c->D);
+        FINALIZE(FIN_ALPHA, n);
+        TIMED_LAUNCH(1, k_bicg_2, n, c->D);
+        FINALIZE(FIN_OMEGA, n);
         LAUNCH(k_bicg_3, n, c->D);
+        FINALIZE(FIN_BETA, n);
         LAUNCH(k_bicg_4, n, c->D);
       }
     }
     it += chunk;
     int rc = read_scalars(c, &s);
     if (rc) return rc;
+    if (c->prof) timers_collect(c);
     if (s.done) break;
     if (chunk < 32) chunk *= 2;
     if (it > c->P.max_iter + 64) break;
@@ -580,8 +687,15 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
   }
   const int n = c->n_hi;
   isph_solve_stats st{};
+  {
+    long long nnz = 0;
+    HIPCHK(c, hipMemcpyAsync(&nnz, &c->D.sc->nnz, sizeof nnz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st.nnz = (double)nnz;
+  }
   // ||b||
   LAUNCH(k_dot, n, c->D, c->D.b, c->D.b, c->d_bn2);
+  FINALIZE_OUT(FIN_DOT, n, c->d_bn2);
   double bn2 = 0;
   HIPCHK(c, hipMemcpyAsync(&bn2, c->d_bn2, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -599,6 +713,7 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
     }
   } else if (c->P.precision == ISPH_PPE_FP64) {
     LAUNCH(k_bicg_init, n, c->D);
+    FINALIZE(FIN_INIT, n);
     rc = read_scalars(c, &s);
     if (rc) return rc;
     int it = 0, brk = 0, conv = s.conv;
@@ -615,6 +730,7 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
     int total = 0;
     for (int outer = 1; outer <= c->P.max_outer; outer++) {
       LAUNCH(k_ref_resid, n, c->D);
+      FINALIZE(FIN_RR, n);
       rc = read_scalars(c, &s);
       if (rc) return rc;
       const double rn = std::sqrt(s.rr);
@@ -625,6 +741,7 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
         break;
       }
       LAUNCH(k_in_init, n, c->D);
+      FINALIZE(FIN_IN_INIT, n);
       int it = 0, brk = 0, conv = 0;
       double rel = 0;
       rc = read_scalars(c, &s);
@@ -638,6 +755,7 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
       LAUNCH(k_ref_update, n, c->D);
       if (outer == c->P.max_outer) {
         LAUNCH(k_ref_resid, n, c->D);
+        FINALIZE(FIN_RR, n);
         rc = read_scalars(c, &s);
         if (rc) return rc;
         st.rel_residual = std::sqrt(s.rr) / bnorm;

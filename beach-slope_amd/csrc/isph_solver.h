@@ -2,10 +2,12 @@
 // reference's bicgstab (solver.f90:182-213) with y = A x in the row order of
 // spmv2 (solver.f90:84-127), plus the FP32-inner / FP64-refinement form.
 //
-// Each BiCGStab iteration is four launches.  Dot products are reduced in a
-// fixed blocked tree (block_tree / final_tree) and the scalars alpha, omega,
-// beta are formed on the device by the last block to finish, so the host
-// only polls the `done` flag every few iterations.
+// Each BiCGStab iteration is four row kernels, three of which write per-block
+// partial dot products (block_tree over 256 rows); a one-block finalize
+// kernel then sums the partials in the fixed order (final_tree) and forms
+// alpha, omega or beta on the device.  The kernel boundary makes the
+// partials visible, so no device-scope fence is needed inside the row
+// kernels.  The host only polls the `done` flag every few iterations.
 #pragma once
 #include "isph_device.h"
 
@@ -36,24 +38,6 @@ __global__ void __launch_bounds__(256) k_bicg_init(Dev D) {
   }
   const double s = block_tree(prod, lds);
   if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-  if (last_block(&D.sc->counter[0])) {
-    const double rr = final_tree(D.part, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      DevScalars* S = D.sc;
-      S->rho = rr;
-      S->rr = rr;
-      S->iters = 0;
-      S->brk = 0;
-      S->conv = 0;
-      S->done = 0;
-      S->rel = sqrt(rr) / S->bnorm;
-      if (sqrt(rr) <= D.P.tol * S->bnorm) {
-        S->conv = 1;
-        S->done = 1;
-      }
-      S->counter[0] = 0;
-    }
-  }
 }
 
 // t1 = A p ; alpha = rho / (t1 . rh)
@@ -70,13 +54,6 @@ __global__ void __launch_bounds__(256) k_bicg_1(Dev D) {
   }
   const double s = block_tree(prod, lds);
   if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-  if (last_block(&D.sc->counter[1])) {
-    const double d = final_tree(D.part, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      D.sc->alpha = D.sc->rho / d;
-      D.sc->counter[1] = 0;
-    }
-  }
 }
 
 // s = r - alpha t1 ; t2 = A s (s at the neighbours formed on the fly) ;
@@ -108,14 +85,6 @@ __global__ void __launch_bounds__(256) k_bicg_2(Dev D) {
     D.part[blockIdx.x] = s1;
     D.part[gridDim.x + blockIdx.x] = s2;
   }
-  if (last_block(&D.sc->counter[2])) {
-    const double d1 = final_tree(D.part, gridDim.x, lds);
-    const double d2 = final_tree(D.part + gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      D.sc->omega = d1 / d2;
-      D.sc->counter[2] = 0;
-    }
-  }
 }
 
 // x = (x + alpha p) + omega s ; r = s - omega t2 ; rho' = r . rh ; rr = r . r
@@ -140,29 +109,6 @@ __global__ void __launch_bounds__(256) k_bicg_3(Dev D) {
     D.part[blockIdx.x] = s1;
     D.part[gridDim.x + blockIdx.x] = s2;
   }
-  if (last_block(&D.sc->counter[3])) {
-    const double rhon = final_tree(D.part, gridDim.x, lds);
-    const double rr = final_tree(D.part + gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      DevScalars* S = D.sc;
-      S->iters = S->iters + 1;
-      S->rr = rr;
-      S->rel = sqrt(rr) / S->bnorm;
-      if (sqrt(rr) <= D.P.tol * S->bnorm) {
-        S->conv = 1;
-        S->done = 1;
-      } else if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 ||
-                 !isfinite(rhon)) {
-        S->brk = 1;
-        S->done = 1;
-      } else {
-        S->beta = (rhon / S->rho) * (alpha / omega);
-        S->rho = rhon;
-        if (S->iters >= D.P.max_iter) S->done = 1;
-      }
-      S->counter[3] = 0;
-    }
-  }
 }
 
 // p = r + beta (p - omega t1)
@@ -183,13 +129,6 @@ __global__ void __launch_bounds__(256) k_dot(Dev D, const double* a, const doubl
   const double prod = (i < n) ? a[i] * b[i] : 0.0;
   const double s = block_tree(prod, lds);
   if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-  if (last_block(&D.sc->counter[0])) {
-    const double d = final_tree(D.part, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      *out = d;
-      D.sc->counter[0] = 0;
-    }
-  }
 }
 
 // pressure scatter (isph.f90:468-486): the solution vector is indexed by row
@@ -213,14 +152,6 @@ __global__ void __launch_bounds__(256) k_ref_resid(Dev D) {
   }
   const double s = block_tree(prod, lds);
   if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-  if (last_block(&D.sc->counter[0])) {
-    const double rr = final_tree(D.part, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      D.sc->rr = rr;
-      D.sc->rel = sqrt(rr) / D.sc->bnorm;
-      D.sc->counter[0] = 0;
-    }
-  }
 }
 
 // inner init: d = 0, r32 = rh32 = p32 = (float) r ; rho = rr0 = r32.r32 (FP64)
@@ -239,18 +170,6 @@ __global__ void __launch_bounds__(256) k_in_init(Dev D) {
   }
   const double s = block_tree(prod, lds);
   if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-  if (last_block(&D.sc->counter[0])) {
-    const double rr = final_tree(D.part, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      DevScalars* S = D.sc;
-      S->rho = rr;
-      S->stop = D.P.inner_tol * sqrt(rr);
-      S->iters = 0;
-      S->brk = 0;
-      S->done = (rr == 0.0);
-      S->counter[0] = 0;
-    }
-  }
 }
 
 __global__ void __launch_bounds__(256) k_in_1(Dev D) {
@@ -266,13 +185,6 @@ __global__ void __launch_bounds__(256) k_in_1(Dev D) {
   }
   const double s = block_tree(prod, lds);
   if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-  if (last_block(&D.sc->counter[1])) {
-    const double d = final_tree(D.part, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      D.sc->alpha = D.sc->rho / d;
-      D.sc->counter[1] = 0;
-    }
-  }
 }
 
 __global__ void __launch_bounds__(256) k_in_2(Dev D) {
@@ -302,14 +214,6 @@ __global__ void __launch_bounds__(256) k_in_2(Dev D) {
     D.part[blockIdx.x] = s1;
     D.part[gridDim.x + blockIdx.x] = s2;
   }
-  if (last_block(&D.sc->counter[2])) {
-    const double d1 = final_tree(D.part, gridDim.x, lds);
-    const double d2 = final_tree(D.part + gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      D.sc->omega = d1 / d2;
-      D.sc->counter[2] = 0;
-    }
-  }
 }
 
 __global__ void __launch_bounds__(256) k_in_3(Dev D) {
@@ -334,26 +238,6 @@ __global__ void __launch_bounds__(256) k_in_3(Dev D) {
     D.part[blockIdx.x] = s1;
     D.part[gridDim.x + blockIdx.x] = s2;
   }
-  if (last_block(&D.sc->counter[3])) {
-    const double rhon = final_tree(D.part, gridDim.x, lds);
-    const double rr = final_tree(D.part + gridDim.x, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-      DevScalars* S = D.sc;
-      S->iters = S->iters + 1;
-      if (sqrt(rr) <= S->stop) {
-        S->done = 1;
-      } else if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 ||
-                 !isfinite(rhon)) {
-        S->brk = 1;
-        S->done = 1;
-      } else {
-        S->beta = (rhon / S->rho) * (alpha / omega);
-        S->rho = rhon;
-        if (S->iters >= D.P.max_iter) S->done = 1;
-      }
-      S->counter[3] = 0;
-    }
-  }
 }
 
 __global__ void k_in_4(Dev D) {
@@ -368,4 +252,120 @@ __global__ void k_in_4(Dev D) {
 __global__ void k_ref_update(Dev D) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < D.sc->n) D.sol[i] = D.sol[i] + (double)D.d32[i];
+}
+
+// ---------------------------------------------------------------------
+// one-block finalize kernels (256 threads) over the partials of the
+// preceding row kernel
+// ---------------------------------------------------------------------
+enum {
+  FIN_INIT = 0,      // rho = rr = sum ; convergence test
+  FIN_ALPHA = 1,     // alpha = rho / sum
+  FIN_OMEGA = 2,     // omega = sum0 / sum1
+  FIN_BETA = 3,      // rho' = sum0, rr = sum1 ; iteration bookkeeping, beta
+  FIN_DOT = 4,       // *out = sum
+  FIN_RR = 5,        // rr = sum (refinement residual)
+  FIN_IN_INIT = 6,   // inner: rho = sum ; stop = inner_tol*sqrt(sum)
+  FIN_IN_ALPHA = 7,  // inner: alpha, breakdown before the update
+  FIN_IN_OMEGA = 8,  // inner: omega, breakdown before the update
+  FIN_IN_BETA = 9    // inner: bookkeeping against `stop`
+};
+
+__global__ void __launch_bounds__(256) k_finalize(Dev D, int op, int nb, double* out) {
+  __shared__ double lds[RED_BLOCK];
+  DevScalars* S = D.sc;
+  const bool skip = (op == FIN_ALPHA || op == FIN_OMEGA || op == FIN_BETA || op == FIN_IN_ALPHA ||
+                     op == FIN_IN_OMEGA || op == FIN_IN_BETA) && S->done;
+  if (skip) return;
+  const double s0 = final_tree(D.part, nb, lds);
+  double s1 = 0.0;
+  if (op == FIN_OMEGA || op == FIN_BETA || op == FIN_IN_OMEGA || op == FIN_IN_BETA)
+    s1 = final_tree(D.part + nb, nb, lds);
+  if (threadIdx.x != 0) return;
+  const DevParams& P = D.P;
+  switch (op) {
+    case FIN_INIT:
+      S->rho = s0;
+      S->rr = s0;
+      S->iters = 0;
+      S->brk = 0;
+      S->conv = 0;
+      S->done = 0;
+      S->rel = sqrt(s0) / S->bnorm;
+      if (sqrt(s0) <= P.tol * S->bnorm) {
+        S->conv = 1;
+        S->done = 1;
+      }
+      break;
+    case FIN_ALPHA:
+      S->alpha = S->rho / s0;
+      break;
+    case FIN_OMEGA:
+      S->omega = s0 / s1;
+      break;
+    case FIN_BETA: {
+      const double alpha = S->alpha, omega = S->omega, rhon = s0, rr = s1;
+      S->iters = S->iters + 1;
+      S->rr = rr;
+      S->rel = sqrt(rr) / S->bnorm;
+      if (sqrt(rr) <= P.tol * S->bnorm) {
+        S->conv = 1;
+        S->done = 1;
+      } else if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 ||
+                 !isfinite(rhon)) {
+        S->brk = 1;
+        S->done = 1;
+      } else {
+        S->beta = (rhon / S->rho) * (alpha / omega);
+        S->rho = rhon;
+        if (S->iters >= P.max_iter) S->done = 1;
+      }
+      break;
+    }
+    case FIN_DOT:
+      *out = s0;
+      break;
+    case FIN_RR:
+      S->rr = s0;
+      S->rel = sqrt(s0) / S->bnorm;
+      break;
+    case FIN_IN_INIT:
+      S->rho = s0;
+      S->stop = P.inner_tol * sqrt(s0);
+      S->iters = 0;
+      S->brk = 0;
+      S->done = (s0 == 0.0);
+      break;
+    case FIN_IN_ALPHA:
+      S->alpha = S->rho / s0;
+      if (!isfinite(S->alpha)) {
+        S->brk = 1;
+        S->done = 1;
+        S->iters = S->iters + 1;
+      }
+      break;
+    case FIN_IN_OMEGA:
+      S->omega = s0 / s1;
+      if (!isfinite(S->omega) || S->omega == 0.0) {
+        S->brk = 1;
+        S->done = 1;
+        S->iters = S->iters + 1;
+      }
+      break;
+    case FIN_IN_BETA: {
+      const double rhon = s0, rr = s1;
+      S->iters = S->iters + 1;
+      if (sqrt(rr) <= S->stop) {
+        S->done = 1;
+      } else if (rhon == 0.0 || !isfinite(rhon)) {
+        S->brk = 1;
+        S->done = 1;
+      } else {
+        S->beta = (rhon / S->rho) * (S->alpha / S->omega);
+        S->rho = rhon;
+        if (S->iters >= P.max_iter) S->done = 1;
+      }
+      break;
+    }
+  }
 }

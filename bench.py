@@ -1,0 +1,162 @@
+"""Benchmark: particle-steps/s of the per-step ISPH pipeline at ~650k particles
+on one MI355X.
+
+A step is one production-loop iteration of the reference (main.f90:199-261)
+over the 650k-particle flume: scalar transport, projection, rebin, neighbour
+search, porosity, SPS, intermediate velocity, PPE assembly and solve,
+pressure-gradient update, trapezoid positions, second rebin and neighbour
+search, free surface, porosity, shifting, density, time step, SPS.  State is
+device resident before the timed region starts.
+
+Two forms are measured on the same input:
+  plain   FP64 BiCGStab (the parity path);
+  chosen  FP32 BiCGStab inner solves + FP64 residual and iterative refinement.
+Both solve to the same relative residual (1e-8) so their throughputs compare
+like for like.  `value` is the chosen form.
+
+The roofline object is for the dominant kernel, the solver's SpMV t = A s
+with its two fused dot products, timed with HIP events on the context's
+stream over the timed region.  The CPU baseline is the oracle (a C/OpenMP
+restatement of the reference) on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode both|plain|chosen]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+import __graft_entry__ as entry
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 650k ISPH; scenarios/h + p99 latency vs plain GPU FP64"
+
+
+def spmv_s_bytes(n: int, nnz: float, fp32: bool) -> float:
+    """Algorithmic bytes of one launch of t = A s (s = r - alpha t1 formed on
+    the fly): row lengths, column indices and values once, r and t1 once,
+    s and t written once."""
+    if fp32:
+        return n * (4 + 8 + 8) + nnz * (4 + 4)
+    return n * (4 + 16 + 16) + nnz * (4 + 8)
+
+
+def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
+    params = dict(case.params)
+    params.update(precision=precision, tol=tol, inner_tol=1e-3, max_outer=20)
+    g = pkg.Scenario(case, params=params)
+    g.init_state()
+    for _ in range(warmup):
+        g.step(1)
+    g.prof_enable(True)
+    iters, outer = [], []
+    nnz = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = g.step(1)
+        iters.append(st.iterations)
+        outer.append(st.outer)
+        nnz = st.nnz
+    t1 = time.perf_counter()
+    kname = "spmv_s32" if precision else "spmv_s"
+    ms, launches = g.prof_read(kname)
+    n = g.n
+    g.prof_enable(False)
+    g.close()
+    wall = t1 - t1 + (t1 - t0)
+    avg_us = 1e3 * ms / max(launches, 1)
+    bytes_per = spmv_s_bytes(n, nnz, bool(precision))
+    achieved = bytes_per / (avg_us * 1e-6) / 1e9 if launches else 0.0
+    return dict(
+        n=n, seconds=wall, ms_per_step=1e3 * wall / steps,
+        particle_steps_per_s=n * steps / wall, iters_mean=float(np.mean(iters)),
+        outer_mean=float(np.mean(outer)), nnz_per_row=nnz / n,
+        kernel=kname, kernel_avg_us=avg_us, kernel_launches=launches,
+        kernel_bytes=bytes_per, kernel_gbs=achieved,
+    )
+
+
+def cpu_baseline(oracle, case, tol: float):
+    """The oracle on one step of the same workload (after its init)."""
+    params = dict(case.params)
+    params.update(precision=0, tol=tol)
+    o = oracle.Oracle(case, params=params)
+    o.call("init_state")
+    t0 = time.perf_counter()
+    rc, st = o.step(1)
+    dt = time.perf_counter() - t0
+    return dict(value=o.n / dt, unit="particle-steps/s", cores=oracle.threads(), kind="port",
+                sample=f"1 step of the same {o.n}-particle workload after init "
+                       f"({st.iterations} BiCGStab iterations, {dt:.1f} s)")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="both", choices=["both", "plain", "chosen"])
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent scenarios would shard one per GPU, see the design notes)",
+              file=sys.stderr)
+        return 2
+    pkg = entry.load_package()
+    from beach_slope_amd import case as cases
+    c = cases.flume_650k()
+    print(f"bench: workload flume_650k n={c.n} steps={args.steps} warmup={args.warmup}",
+          file=sys.stderr, flush=True)
+    res = {}
+    if args.mode in ("both", "plain"):
+        res["plain"] = run_gpu(pkg, c, 0, args.steps, args.warmup, args.tol)
+        print(f"bench: plain {res['plain']}", file=sys.stderr, flush=True)
+    if args.mode in ("both", "chosen"):
+        res["chosen"] = run_gpu(pkg, c, 1, args.steps, args.warmup, args.tol)
+        print(f"bench: chosen {res['chosen']}", file=sys.stderr, flush=True)
+    head = res.get("chosen") or res["plain"]
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(entry.load_oracle(), c, args.tol)
+        print(f"bench: cpu {cpu}", file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC,
+        "value": head["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if "chosen" in res else "f64",
+        "data": "synthetic",
+        "config": {"workload": "flume_650k: 10 m x 0.4 m flume, dx 0.0025, 4000x160 fluid "
+                               "+ 3 wall layers, scalar patch, dt 5e-5",
+                   "particles": head["n"], "ppe_rel_tol": args.tol,
+                   "form": "chosen" if "chosen" in res else "plain"},
+        "roofline": {"bound": "hbm", "achieved": head["kernel_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["kernel_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": head["kernel"],
+                     "avg_launch_us": head["kernel_avg_us"]},
+        "cpu_baseline": cpu,
+    }
+    for k, v in res.items():
+        out[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+    if "plain" in res and "chosen" in res:
+        out["speedup_chosen_vs_plain"] = (res["chosen"]["particle_steps_per_s"] /
+                                          res["plain"]["particle_steps_per_s"])
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -102,6 +102,7 @@ typedef struct isph_solve_stats {
   int32_t breakdown;      /* 1 if rho or omega became zero / non-finite        */
   double rel_residual;    /* final ||b-Ax||/||b|| (recursive for FP64 path)    */
   double bnorm;           /* ||b||                                             */
+  double nnz;             /* stored PPE entries (sum of row lengths)           */
 } isph_solve_stats;
 
 /* Time-stepping scalars (module variables t, dt, dtsol, solsteps, umax, numax). */
@@ -194,6 +195,15 @@ typedef struct isph_batch isph_batch;
 int isph_batch_create(int32_t n_scen, isph_ctx *const *ctxs, isph_batch **out);
 int isph_batch_step(isph_batch *b, int32_t nsteps);
 int isph_batch_destroy(isph_batch *b);
+
+/* ---- kernel timing (HIP events on the context's stream) ----
+ * When enabled, every launch of the solver's two SpMV kernels is bracketed by
+ * a pair of HIP events; isph_prof_read returns the summed event time and the
+ * launch count per kernel.  kernel: "spmv_p" (t = A p), "spmv_s" (t = A s),
+ * "spmv_p32", "spmv_s32" (the FP32 inner solve). */
+int isph_prof_enable(isph_ctx *ctx, int32_t on);
+int isph_prof_read(isph_ctx *ctx, const char *kernel, double *total_ms,
+                   int64_t *launches);
 
 /* Device memory a context of this capacity needs, in bytes (for admission). */
 int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap,

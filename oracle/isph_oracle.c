@@ -38,6 +38,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdio.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -907,10 +908,13 @@ static int bicgstab32(ostate *s, float *d, const float *r0, double itol, int max
     spmv32(s, p, t1);
     double alpha = rho / dot32(s, n, t1, rh, tmp);
     float af = (float)alpha;
+    /* a breakdown stops the inner solve before its update is applied */
+    if (!isfinite(alpha)) { *brk = 1; *iters = it; return 0; }
     for (int64_t i = 0; i < n; i++) sv[i] = r[i] - af * t1[i];
     spmv32(s, sv, t2);
     double omega = dot32(s, n, t2, sv, tmp) / dot32(s, n, t2, t2, tmp);
     float of = (float)omega;
+    if (!isfinite(omega) || omega == 0.0) { *brk = 1; *iters = it; return 0; }
     for (int64_t i = 0; i < n; i++) {
       d[i] = (d[i] + af * p[i]) + of * sv[i];
       r[i] = sv[i] - of * t2[i];
@@ -919,7 +923,7 @@ static int bicgstab32(ostate *s, float *d, const float *r0, double itol, int max
     double rr = dot32(s, n, r, r, tmp);
     *iters = it;
     if (sqrt(rr) <= stop) return 0;
-    if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 || !isfinite(rhon)) {
+    if (rhon == 0.0 || !isfinite(rhon)) {
       *brk = 1;
       return 0;
     }
@@ -938,6 +942,7 @@ int or_ppe_solve(ostate *s, isph_solve_stats *out) {
   double bn2 = dot(s, n, s->b, s->b, s->w8);
   double bnorm = sqrt(bn2);
   st.bnorm = bnorm;
+  for (int64_t i = 0; i < n; i++) st.nnz += s->alen[i];
   if (bnorm == 0.0 || !isfinite(bnorm)) {
     if (bnorm == 0.0) {
       for (int64_t i = 0; i < n; i++) s->sol[i] = 0.0;
@@ -961,6 +966,7 @@ int or_ppe_solve(ostate *s, isph_solve_stats *out) {
       for (int64_t i = 0; i < n; i++) r32[i] = (float)r[i];
       int it = 0, brk = 0;
       bicgstab32(s, dcor, r32, s->P.inner_tol, s->P.max_iter, &it, &brk);
+      if (getenv("ORACLE_DEBUG")) fprintf(stderr, "outer %d rn/bn %.3e inner it %d brk %d\n", outer, rn / bnorm, it, brk);
       total += it;
       if (brk) st.breakdown = 1;
       for (int64_t i = 0; i < n; i++) s->sol[i] = s->sol[i] + (double)dcor[i];

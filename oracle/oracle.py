@@ -34,7 +34,8 @@ class IsphParams(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32),
-                ("breakdown", C.c_int32), ("rel_residual", C.c_double), ("bnorm", C.c_double)]
+                ("breakdown", C.c_int32), ("rel_residual", C.c_double), ("bnorm", C.c_double),
+                ("nnz", C.c_double)]
 
 
 class Scalars(C.Structure):
@@ -131,7 +132,10 @@ class Oracle:
         self.cap = cap or case.n
         self.s = L.or_create(C.byref(self.P), self.cap, max(case.ns, 1))
         self._keep = []
-        parts = {k: np.ascontiguousarray(v) for k, v in case.parts.items()}
+        parts = {k: np.ascontiguousarray(case.parts[k], dtype=np.float64) for k in PART_FIELDS}
+        parts["tid"] = np.ascontiguousarray(case.parts["tid"], dtype=np.int32)
+        parts["flags"] = np.ascontiguousarray(case.parts["flags"], dtype=np.uint8)
+        self._keep = parts
         args = [_ptr(parts[k]) for k in PART_FIELDS] + [_ptr(parts["tid"]), _ptr(parts["flags"])]
         rc = L.or_upload(self.s, case.n, *args)
         if rc != 0:
