@@ -16,6 +16,7 @@
 #include "isph_device.h"
 #include "isph_kernels.h"
 #include "isph_solver.h"
+#include "isph_mg.h"
 
 static_assert(NBR_CAP == ISPH_NBR_CAP, "neighbour capacity mismatch");
 
@@ -42,6 +43,8 @@ struct isph_ctx {
   std::string err;
   std::vector<void*> allocs;
   double* d_bn2 = nullptr;
+  MgDev G{};               // multigrid hierarchy (precision form)
+  double* mg_work = nullptr;
 };
 
 #define HIPCHK(ctx, expr)                                                     \
@@ -177,12 +180,37 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
     rc |= dalloc(c, &D.aval32, N * ROW_CAP);
     float** ff[] = {&D.r32, &D.rh32, &D.pp32, &D.t132, &D.sv32, &D.t232, &D.d32};
     for (auto p : ff) rc |= dalloc(c, p, N);
+    // cell hierarchy: level 1 is the cell grid, then 2x2 aggregation down to
+    // at most MG_DENSE_MAX unknowns
+    MgDev& G = c->G;
+    int ny = params->celly, nx = params->cellx, off = 0, l = 1;
+    for (;;) {
+      if (l >= MG_MAX_LEVELS) { rc |= ISPH_EARG; break; }
+      G.lev[l].ny = ny; G.lev[l].nx = nx; G.lev[l].n = ny * nx; G.lev[l].off = off;
+      off += ny * nx;
+      if (ny * nx <= MG_DENSE_MAX) break;
+      ny = (ny + 1) / 2; nx = (nx + 1) / 2; l++;
+    }
+    G.nlev = l;
+    rc |= dalloc(c, &G.sten, 9 * (int64_t)off);
+    rc |= dalloc(c, &G.xb, off);
+    rc |= dalloc(c, &G.bb, off);
+    rc |= dalloc(c, &G.rb, off);
+    rc |= dalloc(c, &G.dense_inv, MG_DENSE_MAX * MG_DENSE_MAX);
+    rc |= dalloc(c, &c->mg_work, 2 * MG_DENSE_MAX * MG_DENSE_MAX);
+    rc |= dalloc(c, &G.d0, N);
+    rc |= dalloc(c, &G.x0, N);
+    G.w = 0.7f;
+    G.cscale = 1.6f;
   }
   if (rc) {
     std::string e = c->err;
     isph_destroy(c);
     return ISPH_EHIP;
   }
+  c->G.done = &D.sc->done;
+  if (c->G.nlev >= 1)
+    c->G.e1 = (c->G.nlev >= 2) ? c->G.rb + c->G.lev[1].off : c->G.xb + c->G.lev[1].off;
   DevScalars s{};
   s.dt = params->dt;
   s.dtsol = params->dt;
@@ -639,31 +667,116 @@ extern "C" int isph_ppe_assemble(isph_ctx* c) {
   return check_err_flags(c);
 }
 
+// one V-cycle of the cell multigrid: out = M(v)
+template <typename VT>
+static int mg_apply(isph_ctx* c, const VT* v, float* out) {
+  const int n = c->n_hi;
+  MgDev& G = c->G;
+  hipLaunchKernelGGL(k_mg_pre0<VT>, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, G, v,
+                     c->D.d32);
+  hipLaunchKernelGGL(k_mg_restrict0, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, c->D, G,
+                     (const float*)c->D.d32);
+  if (G.nlev >= 2) {
+    hipLaunchKernelGGL(k_mg_l1_down, dim3(grid_of(G.lev[2].n)), dim3(256), 0, c->stream, G);
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_CO_THREADS), 0, c->stream, G, 2);
+    hipLaunchKernelGGL(k_mg_l1_up, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, G);
+  } else {
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_CO_THREADS), 0, c->stream, G, 1);
+  }
+  hipLaunchKernelGGL(k_mg_post0<VT>, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, G, v, out);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// hierarchy of the precision form, rebuilt from the assembled FP64 rows
+static int mg_setup(isph_ctx* c) {
+  MgDev& G = c->G;
+  hipLaunchKernelGGL(k_mg_setup0, dim3(grid_of(c->n_hi)), dim3(256), 0, c->stream, c->D, G);
+  hipLaunchKernelGGL(k_mg_stencil1, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, c->D, G);
+  for (int l = 1; l < G.nlev; l++)
+    hipLaunchKernelGGL(k_mg_coarsen, dim3(grid_of(G.lev[l + 1].n)), dim3(256), 0, c->stream, G, l);
+  hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(256), 0, c->stream, G, c->mg_work);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+#define PFINALIZE(op, nrows, init)                                                   \
+  do {                                                                               \
+    hipLaunchKernelGGL(k_pfinalize, dim3(1), dim3(256), 0, c->stream, c->D, (op),    \
+                       grid_of(nrows), (init));                                      \
+    HIPCHK(c, hipGetLastError());                                                    \
+  } while (0)
+
+// precision form: cell-multigrid right-preconditioned BiCGStab with FP32
+// matrix, V-cycles and search directions; FP64 solution, residual and dots.
+// A recursive-residual convergence claim is checked against b - A64 x; when
+// it fails the iteration restarts from the FP64 residual.
+static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
+  const int n = c->n_hi;
+  int rc = mg_setup(c);
+  if (rc) return rc;
+  float* ph = c->D.r32;
+  float* sh = c->D.sv32;
+  hipLaunchKernelGGL(k_pb_resid, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, 1);
+  PFINALIZE(PFIN_RESID, n, 1);
+  DevScalars s;
+  rc = read_scalars(c, &s);
+  if (rc) return rc;
+  int restarts = 0, chunk = 4;
+  while (!s.conv) {
+    for (int k = 0; k < chunk; k++) {
+      rc = mg_apply<float>(c, c->D.pp32, ph);
+      if (rc) return rc;
+      TIMED_LAUNCH(2, k_pb_1, n, c->D, c->G, (const float*)ph);
+      PFINALIZE(PFIN_ALPHA, n, 0);
+      LAUNCH(k_pb_s, n, c->D);
+      rc = mg_apply<double>(c, c->D.sv, sh);
+      if (rc) return rc;
+      TIMED_LAUNCH(3, k_pb_2, n, c->D, c->G, (const float*)sh);
+      PFINALIZE(PFIN_OMEGA, n, 0);
+      LAUNCH(k_pb_3, n, c->D, (const float*)ph, (const float*)sh);
+      PFINALIZE(PFIN_BETA, n, 0);
+      LAUNCH(k_pb_4, n, c->D);
+    }
+    rc = read_scalars(c, &s);
+    if (rc) return rc;
+    if (c->prof) timers_collect(c);
+    if (!s.done) {
+      if (chunk < 16) chunk *= 2;
+      continue;
+    }
+    // the FP64 residual decides; restart from it unless it is met
+    hipLaunchKernelGGL(k_pb_resid, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, 2);
+    PFINALIZE(PFIN_RESID, n, 2);
+    rc = read_scalars(c, &s);
+    if (rc) return rc;
+    if (s.conv) break;
+    if (s.iters >= c->P.max_iter || ++restarts > c->P.max_outer) break;
+    chunk = 4;
+  }
+  st->iterations = s.iters;
+  st->outer = restarts;
+  st->converged = s.conv;
+  st->breakdown = s.brk;
+  st->rel_residual = s.rel;
+  return 0;
+}
+
 // run BiCGStab chunks until the device reports done
-static int bicg_loop(isph_ctx* c, bool inner, int* iters_out, int* brk_out, int* conv_out,
+static int bicg_loop(isph_ctx* c, int* iters_out, int* brk_out, int* conv_out,
                      double* rel_out) {
   const int n = c->n_hi;
   int chunk = 4;
   DevScalars s;
   for (int it = 0;; ) {
     for (int k = 0; k < chunk; k++) {
-      if (inner) {
-        TIMED_LAUNCH(2, k_in_1, n, c->D);
-        FINALIZE(FIN_IN_ALPHA, n);
-        TIMED_LAUNCH(3, k_in_2, n, c->D);
-        FINALIZE(FIN_IN_OMEGA, n);
-        LAUNCH(k_in_3, n, c->D);
-        FINALIZE(FIN_IN_BETA, n);
-        LAUNCH(k_in_4, n, c->D);
-      } else {
-        TIMED_LAUNCH(0, k_bicg_1, n, c->D);
-        FINALIZE(FIN_ALPHA, n);
-        TIMED_LAUNCH(1, k_bicg_2, n, c->D);
-        FINALIZE(FIN_OMEGA, n);
-        LAUNCH(k_bicg_3, n, c->D);
-        FINALIZE(FIN_BETA, n);
-        LAUNCH(k_bicg_4, n, c->D);
-      }
+      TIMED_LAUNCH(0, k_bicg_1, n, c->D);
+      FINALIZE(FIN_ALPHA, n);
+      TIMED_LAUNCH(1, k_bicg_2, n, c->D);
+      FINALIZE(FIN_OMEGA, n);
+      LAUNCH(k_bicg_3, n, c->D);
+      FINALIZE(FIN_BETA, n);
+      LAUNCH(k_bicg_4, n, c->D);
     }
     it += chunk;
     int rc = read_scalars(c, &s);
@@ -719,7 +832,7 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
     int it = 0, brk = 0, conv = s.conv;
     double rel = s.rel;
     if (!s.done) {
-      rc = bicg_loop(c, false, &it, &brk, &conv, &rel);
+      rc = bicg_loop(c, &it, &brk, &conv, &rel);
       if (rc) return rc;
     }
     st.iterations = it;
@@ -727,43 +840,8 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
     st.converged = conv;
     st.rel_residual = rel;
   } else {
-    int total = 0;
-    for (int outer = 1; outer <= c->P.max_outer; outer++) {
-      LAUNCH(k_ref_resid, n, c->D);
-      FINALIZE(FIN_RR, n);
-      rc = read_scalars(c, &s);
-      if (rc) return rc;
-      const double rn = std::sqrt(s.rr);
-      st.rel_residual = rn / bnorm;
-      st.outer = outer - 1;
-      if (rn <= c->P.tol * bnorm) {
-        st.converged = 1;
-        break;
-      }
-      LAUNCH(k_in_init, n, c->D);
-      FINALIZE(FIN_IN_INIT, n);
-      int it = 0, brk = 0, conv = 0;
-      double rel = 0;
-      rc = read_scalars(c, &s);
-      if (rc) return rc;
-      if (!s.done) {
-        rc = bicg_loop(c, true, &it, &brk, &conv, &rel);
-        if (rc) return rc;
-      }
-      total += it;
-      if (brk) st.breakdown = 1;
-      LAUNCH(k_ref_update, n, c->D);
-      if (outer == c->P.max_outer) {
-        LAUNCH(k_ref_resid, n, c->D);
-        FINALIZE(FIN_RR, n);
-        rc = read_scalars(c, &s);
-        if (rc) return rc;
-        st.rel_residual = std::sqrt(s.rr) / bnorm;
-        st.outer = outer;
-        st.converged = std::sqrt(s.rr) <= c->P.tol * bnorm;
-      }
-    }
-    st.iterations = total;
+    rc = pbicg_solve(c, &st);
+    if (rc) return rc;
   }
   LAUNCH(k_copy_sol_to_p, n, c->D);
   c->last = st;

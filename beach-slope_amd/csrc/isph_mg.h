@@ -1,0 +1,525 @@
+// isph_mg.h -- cell-aggregation multigrid preconditioner for the precision
+// form of the PPE solve.
+//
+// Level 0 is the particle matrix itself (FP32 copy of the canonical rows).
+// Level 1 aggregates every cell's particles into one unknown: its operator is
+// the Galerkin sum A1(I,J) = sum_{i in I, j in J} A_ij, a 9-point stencil on
+// the regular cell grid (neighbours lie within 2h < 4h, so only adjacent
+// cells couple).  Level l+1 aggregates 2x2 blocks of level l, again a 9-point
+// stencil.  The first level with at most MG_DENSE_MAX unknowns is solved with
+// a precomputed explicit inverse.
+//
+// Because particles are sorted by cell, each cell's rows are one contiguous
+// range, so every Galerkin sum and every restriction is a sequential loop of
+// one thread over that range: deterministic, no atomics.
+//
+// One V-cycle M(v): x0 = w v/d; r0 = v - A x0; r1 = R r0; e1 = V-cycle on the
+// cell levels; x1 = x0 + c P e1; x = x1 + w (v - A x1)/d, with damped-Jacobi
+// weight w and coarse-correction scale c.  The cell levels (>= 1) run inside
+// one single-workgroup kernel, so a V-cycle is four launches.
+#pragma once
+#include "isph_device.h"
+
+#define MG_MAX_LEVELS 24
+#define MG_DENSE_MAX 128
+#define MG_CO_THREADS 1024
+
+struct MgLevel {
+  int ny, nx, n;   // grid rows (ci), columns (cj), unknowns n = ny*nx
+  int off;         // offset of this level's vectors / stencil in the pools
+};
+
+struct MgDev {
+  int nlev;                    // cell levels 1..nlev (index 0 unused)
+  MgLevel lev[MG_MAX_LEVELS];
+  float* sten;                 // 9 * (sum of n) stencil pool, [k*n_l + I] at off*9
+  float* xb;                   // per-level solution pool
+  float* bb;                   // per-level right-hand side pool
+  float* rb;                   // per-level residual pool
+  double* dense_inv;           // MG_DENSE_MAX^2 explicit inverse of the coarsest level
+  float* d0;                   // level 0: 1 / (sum of the row's entries in column i)
+  float* x0;                   // level 0 pre-smoothed iterate
+  float w, cscale;
+  const int* done;             // solver's done flag: V-cycles after convergence are no-ops
+  float* e1;                   // final level-1 correction read by the level-0 post-smoother
+};
+
+__device__ __forceinline__ int mg_nbr(const MgLevel& L, int I, int k) {
+  const int cj = I / L.ny, ci = I - cj * L.ny;
+  const int di = (k % 3) - 1, dj = (k / 3) - 1;
+  const int a = ci + di, b = cj + dj;
+  if (a < 0 || a >= L.ny || b < 0 || b >= L.nx) return -1;
+  return b * L.ny + a;
+}
+
+// ---- setup ----
+// level 0: inverse row diagonal (diagonal plus the slot-1 value of ghost rows)
+__global__ void k_mg_setup0(Dev D, MgDev G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.sc->n) return;
+  const int cap = D.cap, L = D.alen[i];
+  double d = 0.0;
+  for (int k = 0; k < L; k++)
+    if (D.acol[k * cap + i] == i) d = d + D.aval[k * cap + i];
+  G.d0[i] = (d != 0.0 && isfinite(d)) ? (float)(1.0 / d) : 1.0f;
+}
+
+// level 1: Galerkin sums over each cell's particle rows
+__global__ void k_mg_stencil1(Dev D, MgDev G) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const MgLevel L = G.lev[1];
+  if (I >= L.n) return;
+  double s[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) s[k] = 0.0;
+  const int cjI = I / L.ny, ciI = I - cjI * L.ny;
+  const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
+  const int cap = D.cap;
+  for (int i = i0; i < i1; i++) {
+    const int Lr = D.alen[i];
+    for (int k = 0; k < Lr; k++) {
+      const int j = D.acol[k * cap + i];
+      const int J = D.cell[j];
+      const int cjJ = J / L.ny, ciJ = J - cjJ * L.ny;
+      const int di = ciJ - ciI, dj = cjJ - cjI;
+      if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;  // a lagging particle
+      s[(dj + 1) * 3 + (di + 1)] += D.aval[k * cap + i];
+    }
+  }
+  float* st = G.sten + 9 * L.off;
+  if (i1 == i0) s[4] = 1.0;  // empty cell: identity row
+#pragma unroll
+  for (int k = 0; k < 9; k++) st[k * L.n + I] = (float)s[k];
+}
+
+// level l+1 from level l: sum the 2x2 children's stencils
+__global__ void k_mg_coarsen(MgDev G, int l) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const MgLevel F = G.lev[l], C = G.lev[l + 1];
+  if (I >= C.n) return;
+  const int CJ = I / C.ny, CI = I - CJ * C.ny;
+  const float* sf = G.sten + 9 * F.off;
+  float s[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) s[k] = 0.0f;
+  int nchild = 0;
+  for (int b = 0; b < 2; b++) {
+    for (int a = 0; a < 2; a++) {
+      const int ci = 2 * CI + a, cj = 2 * CJ + b;
+      if (ci >= F.ny || cj >= F.nx) continue;
+      nchild++;
+      const int f = cj * F.ny + ci;
+      for (int k = 0; k < 9; k++) {
+        const int di = (k % 3) - 1, dj = (k / 3) - 1;
+        const int ni = ci + di, nj = cj + dj;
+        if (ni < 0 || ni >= F.ny || nj < 0 || nj >= F.nx) continue;
+        const int pi = (ni >> 1) - CI, pj = (nj >> 1) - CJ;
+        s[(pj + 1) * 3 + (pi + 1)] += sf[k * F.n + f];
+      }
+    }
+  }
+  if (s[4] == 0.0f) s[4] = 1.0f;
+  float* sc = G.sten + 9 * C.off;
+#pragma unroll
+  for (int k = 0; k < 9; k++) sc[k * C.n + I] = s[k];
+}
+
+// coarsest level: dense matrix and its explicit inverse by Gauss-Jordan with
+// partial pivoting (single workgroup)
+__global__ void __launch_bounds__(256) k_mg_dense(MgDev G, double* work) {
+  const MgLevel L = G.lev[G.nlev];
+  const int n = L.n, t = threadIdx.x;
+  const float* st = G.sten + 9 * L.off;
+  double* A = work;                    // n x 2n augmented [A | I]
+  const int w2 = 2 * n;
+  for (int e = t; e < n * w2; e += blockDim.x) A[e] = 0.0;
+  __syncthreads();
+  for (int I = t; I < n; I += blockDim.x) {
+    for (int k = 0; k < 9; k++) {
+      const int J = mg_nbr(L, I, k);
+      if (J >= 0) A[I * w2 + J] += (double)st[k * n + I];
+    }
+    A[I * w2 + n + I] = 1.0;
+  }
+  __syncthreads();
+  __shared__ int piv;
+  __shared__ double colc[MG_DENSE_MAX];
+  for (int c = 0; c < n; c++) {
+    if (t == 0) {
+      int best = c;
+      double bv = fabs(A[c * w2 + c]);
+      for (int r = c + 1; r < n; r++)
+        if (fabs(A[r * w2 + c]) > bv) { bv = fabs(A[r * w2 + c]); best = r; }
+      piv = best;
+    }
+    __syncthreads();
+    if (piv != c)
+      for (int e = t; e < w2; e += blockDim.x) {
+        double tmp = A[c * w2 + e];
+        A[c * w2 + e] = A[piv * w2 + e];
+        A[piv * w2 + e] = tmp;
+      }
+    __syncthreads();
+    const double pv = A[c * w2 + c];
+    __syncthreads();
+    for (int e = t; e < w2; e += blockDim.x) A[c * w2 + e] = (pv != 0.0) ? A[c * w2 + e] / pv : 0.0;
+    // snapshot the pivot column before any row is updated
+    for (int r = t; r < n; r += blockDim.x) colc[r] = A[r * w2 + c];
+    __syncthreads();
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      const double f = colc[r];
+      if (f == 0.0) continue;
+      for (int e = t; e < w2; e += blockDim.x) A[r * w2 + e] = A[r * w2 + e] - f * A[c * w2 + e];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < n * n; e += blockDim.x) {
+    const int r = e / n, cc = e - r * n;
+    G.dense_inv[e] = A[r * w2 + n + cc];
+  }
+}
+
+// ---- V-cycle, level 0 ----
+// x0 = w v d ; r0 = v - A x0 (x0 of the neighbours formed on the fly)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_mg_pre0(Dev D, MgDev G, const VT* __restrict__ v,
+                                                 float* __restrict__ r0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*G.done || i >= D.sc->n) return;
+  const int cap = D.cap, L = D.alen[i];
+  const float w = G.w;
+  float acc = 0.0f;
+  for (int k = 0; k < L; k++) {
+    const int j = D.acol[k * cap + i];
+    acc = acc + D.aval32[k * cap + i] * (w * (float)v[j] * G.d0[j]);
+  }
+  const float vi = (float)v[i];
+  G.x0[i] = w * vi * G.d0[i];
+  r0[i] = vi - acc;
+}
+
+// r1 = R r0: sum of each cell's contiguous range
+__global__ void k_mg_restrict0(Dev D, MgDev G, const float* __restrict__ r0) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const MgLevel L = G.lev[1];
+  if (*G.done || I >= L.n) return;
+  float s = 0.0f;
+  const int i1 = D.cstart[I + 1];
+  for (int i = D.cstart[I]; i < i1; i++) s = s + r0[i];
+  G.bb[L.off + I] = s;
+}
+
+// x = x1 + w (v - A x1) d with x1 = x0 + c P e1 (x1 of the neighbours on the fly)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_mg_post0(Dev D, MgDev G, const VT* __restrict__ v,
+                                                  float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*G.done || i >= D.sc->n) return;
+  const int cap = D.cap, L = D.alen[i];
+  const float c = G.cscale;
+  const float* e1 = G.e1;
+  float acc = 0.0f;
+  for (int k = 0; k < L; k++) {
+    const int j = D.acol[k * cap + i];
+    acc = acc + D.aval32[k * cap + i] * (G.x0[j] + c * e1[D.cell[j]]);
+  }
+  const float x1 = G.x0[i] + c * e1[D.cell[i]];
+  out[i] = x1 + G.w * ((float)v[i] - acc) * G.d0[i];
+}
+
+// ---- V-cycle on the cell levels, one workgroup ----
+__device__ __forceinline__ float mg_apply_row(const MgLevel& L, const float* st, const float* x,
+                                              int I) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int J = mg_nbr(L, I, k);
+    if (J >= 0) acc = acc + st[k * L.n + I] * x[J];
+  }
+  return acc;
+}
+
+// levels l0..nlev in one workgroup (l0 = 2 when level 1 runs multi-block)
+__global__ void __launch_bounds__(MG_CO_THREADS) k_mg_coarse(MgDev G, int l0) {
+  if (*G.done) return;
+  const int t = threadIdx.x, T = blockDim.x;
+  const int nl = G.nlev;
+  const float w = G.w, c = G.cscale;
+  // down: pre-smooth, residual, restrict
+  for (int l = l0; l < nl; l++) {
+    const MgLevel L = G.lev[l];
+    const float* st = G.sten + 9 * L.off;
+    float* x = G.xb + L.off;
+    const float* b = G.bb + L.off;
+    float* r = G.rb + L.off;
+    for (int I = t; I < L.n; I += T) x[I] = w * b[I] / st[4 * L.n + I];
+    __syncthreads();
+    for (int I = t; I < L.n; I += T) r[I] = b[I] - mg_apply_row(L, st, x, I);
+    __syncthreads();
+    const MgLevel C = G.lev[l + 1];
+    float* bc = G.bb + C.off;
+    for (int J = t; J < C.n; J += T) {
+      const int CJ = J / C.ny, CI = J - CJ * C.ny;
+      float s = 0.0f;
+      for (int bq = 0; bq < 2; bq++)
+        for (int a = 0; a < 2; a++) {
+          const int ci = 2 * CI + a, cj = 2 * CJ + bq;
+          if (ci < L.ny && cj < L.nx) s = s + r[cj * L.ny + ci];
+        }
+      bc[J] = s;
+    }
+    __syncthreads();
+  }
+  // coarsest: explicit inverse
+  {
+    const MgLevel L = G.lev[nl];
+    const float* b = G.bb + L.off;
+    float* x = G.xb + L.off;
+    for (int I = t; I < L.n; I += T) {
+      double acc = 0.0;
+      for (int J = 0; J < L.n; J++) acc = acc + G.dense_inv[I * L.n + J] * (double)b[J];
+      x[I] = (float)acc;
+    }
+    __syncthreads();
+  }
+  // up: prolong, correct, post-smooth
+  for (int l = nl - 1; l >= l0; l--) {
+    const MgLevel L = G.lev[l], C = G.lev[l + 1];
+    const float* st = G.sten + 9 * L.off;
+    float* x = G.xb + L.off;
+    const float* b = G.bb + L.off;
+    const float* xc = G.xb + C.off;
+    for (int I = t; I < L.n; I += T) {
+      const int cj = I / L.ny, ci = I - cj * L.ny;
+      x[I] = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+    }
+    __syncthreads();
+    float* r = G.rb + L.off;
+    for (int I = t; I < L.n; I += T) r[I] = b[I] - mg_apply_row(L, st, x, I);
+    __syncthreads();
+    for (int I = t; I < L.n; I += T) x[I] = x[I] + w * r[I] / st[4 * L.n + I];
+    __syncthreads();
+  }
+}
+
+// level 1 on many workgroups.  Down: one thread per level-2 aggregate
+// pre-smooths its (up to four) cells, forms their residuals with the
+// neighbours' pre-smoothed values computed on the fly, and restricts.
+__global__ void k_mg_l1_down(MgDev G) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+  const MgLevel L = G.lev[1], C = G.lev[2];
+  if (*G.done || J >= C.n) return;
+  const float w = G.w;
+  const float* st = G.sten + 9 * L.off;
+  const float* b = G.bb + L.off;
+  float* x = G.xb + L.off;
+  const int CJ = J / C.ny, CI = J - CJ * C.ny;
+  float s = 0.0f;
+  for (int bq = 0; bq < 2; bq++)
+    for (int a = 0; a < 2; a++) {
+      const int ci = 2 * CI + a, cj = 2 * CJ + bq;
+      if (ci >= L.ny || cj >= L.nx) continue;
+      const int I = cj * L.ny + ci;
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int N = mg_nbr(L, I, k);
+        if (N >= 0) acc = acc + st[k * L.n + I] * (w * b[N] / st[4 * L.n + N]);
+      }
+      x[I] = w * b[I] / st[4 * L.n + I];
+      s = s + (b[I] - acc);
+    }
+  G.bb[C.off + J] = s;
+}
+
+// Up: one thread per cell adds the prolonged level-2 correction and
+// post-smooths, with the neighbours' corrected values formed on the fly; the
+// result goes to e1 (the level-1 residual slot, free by then).
+__global__ void k_mg_l1_up(MgDev G) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const MgLevel L = G.lev[1], C = G.lev[2];
+  if (*G.done || I >= L.n) return;
+  const float w = G.w, c = G.cscale;
+  const float* st = G.sten + 9 * L.off;
+  const float* b = G.bb + L.off;
+  const float* x = G.xb + L.off;
+  const float* xc = G.xb + C.off;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int N = mg_nbr(L, I, k);
+    if (N >= 0) {
+      const int nj = N / L.ny, ni = N - nj * L.ny;
+      acc = acc + st[k * L.n + I] * (x[N] + c * xc[(nj >> 1) * C.ny + (ni >> 1)]);
+    }
+  }
+  const int cj = I / L.ny, ci = I - cj * L.ny;
+  const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+  G.e1[I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+}
+
+// ---- preconditioned mixed-precision BiCGStab ----
+// ph = M(p), t1 = A32 ph ; partial (t1 . rh)
+__global__ void __launch_bounds__(256) k_pb_1(Dev D, MgDev G, const float* __restrict__ ph) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double prod = 0.0;
+  if (i < n) {
+    const int cap = D.cap, L = D.alen[i];
+    float acc = 0.0f;
+    for (int k = 0; k < L; k++) acc = acc + D.aval32[k * cap + i] * ph[D.acol[k * cap + i]];
+    D.t132[i] = acc;
+    prod = (double)acc * (double)D.rh32[i];
+  }
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+}
+
+// s = r - alpha t1 (FP64 and an FP32 copy for the V-cycle)
+__global__ void k_pb_s(Dev D) {
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.sc->n) return;
+  const double s = D.r[i] - D.sc->alpha * (double)D.t132[i];
+  D.sv[i] = s;
+}
+
+// t2 = A32 sh ; partials (t2 . s), (t2 . t2)
+__global__ void __launch_bounds__(256) k_pb_2(Dev D, MgDev G, const float* __restrict__ sh) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double p1 = 0.0, p2 = 0.0;
+  if (i < n) {
+    const int cap = D.cap, L = D.alen[i];
+    float acc = 0.0f;
+    for (int k = 0; k < L; k++) acc = acc + D.aval32[k * cap + i] * sh[D.acol[k * cap + i]];
+    D.t232[i] = acc;
+    p1 = (double)acc * D.sv[i];
+    p2 = (double)acc * (double)acc;
+  }
+  const double s1 = block_tree(p1, lds);
+  const double s2 = block_tree(p2, lds);
+  if (threadIdx.x == 0) {
+    D.part[blockIdx.x] = s1;
+    D.part[gridDim.x + blockIdx.x] = s2;
+  }
+}
+
+// x += alpha ph + omega sh (FP64) ; r = s - omega t2 ; partials (r . rh), (r . r)
+__global__ void __launch_bounds__(256) k_pb_3(Dev D, const float* __restrict__ ph,
+                                              const float* __restrict__ sh) {
+  __shared__ double lds[RED_BLOCK];
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  const double alpha = D.sc->alpha, omega = D.sc->omega;
+  double p1 = 0.0, p2 = 0.0;
+  if (i < n) {
+    D.sol[i] = (D.sol[i] + alpha * (double)ph[i]) + omega * (double)sh[i];
+    const double r = D.sv[i] - omega * (double)D.t232[i];
+    D.r[i] = r;
+    p1 = r * (double)D.rh32[i];
+    p2 = r * r;
+  }
+  const double s1 = block_tree(p1, lds);
+  const double s2 = block_tree(p2, lds);
+  if (threadIdx.x == 0) {
+    D.part[blockIdx.x] = s1;
+    D.part[gridDim.x + blockIdx.x] = s2;
+  }
+}
+
+// p = r + beta (p - omega t1)  (p stored FP32)
+__global__ void k_pb_4(Dev D) {
+  if (D.sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.sc->n) return;
+  const double beta = D.sc->beta, omega = D.sc->omega;
+  D.pp32[i] = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * (double)D.t132[i]));
+}
+
+// r = b - A64 x (FP64 true residual); on init also rh32 = p32 = (float) r
+__global__ void __launch_bounds__(256) k_pb_resid(Dev D, int init) {
+  __shared__ double lds[RED_BLOCK];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  double p1 = 0.0, p2 = 0.0;
+  if (i < n) {
+    const int cap = D.cap, L = D.alen[i];
+    double acc = 0.0;
+    for (int k = 0; k < L; k++) acc = acc + D.aval[k * cap + i] * D.sol[D.acol[k * cap + i]];
+    const double r = D.b[i] - acc;
+    D.r[i] = r;
+    if (init) {
+      D.rh32[i] = (float)r;
+      D.pp32[i] = (float)r;
+    }
+    p1 = r * (double)(init ? (float)r : D.rh32[i]);
+    p2 = r * r;
+  }
+  const double s1 = block_tree(p1, lds);
+  const double s2 = block_tree(p2, lds);
+  if (threadIdx.x == 0) {
+    D.part[blockIdx.x] = s1;
+    D.part[gridDim.x + blockIdx.x] = s2;
+  }
+}
+
+enum { PFIN_ALPHA = 0, PFIN_OMEGA = 1, PFIN_BETA = 2, PFIN_RESID = 3 };
+
+// one-block finalize for the preconditioned solve.  PFIN_BETA stops on the
+// recursive residual only as a candidate: the host then checks the FP64
+// residual (PFIN_RESID) and continues if it is not met.
+__global__ void __launch_bounds__(256) k_pfinalize(Dev D, int op, int nb, int init) {
+  __shared__ double lds[RED_BLOCK];
+  DevScalars* S = D.sc;
+  if ((op == PFIN_ALPHA || op == PFIN_OMEGA || op == PFIN_BETA) && S->done) return;
+  const double s0 = final_tree(D.part, nb, lds);
+  double s1 = 0.0;
+  if (op != PFIN_ALPHA) s1 = final_tree(D.part + nb, nb, lds);
+  if (threadIdx.x != 0) return;
+  const DevParams& P = D.P;
+  switch (op) {
+    case PFIN_ALPHA:
+      S->alpha = S->rho / s0;
+      if (!isfinite(S->alpha)) { S->brk = 1; S->done = 1; }
+      break;
+    case PFIN_OMEGA:
+      S->omega = s0 / s1;
+      if (!isfinite(S->omega) || S->omega == 0.0) { S->brk = 1; S->done = 1; }
+      break;
+    case PFIN_BETA: {
+      const double rhon = s0, rr = s1;
+      S->iters = S->iters + 1;
+      S->rr = rr;
+      S->rel = sqrt(rr) / S->bnorm;
+      if (sqrt(rr) <= 0.25 * P.tol * S->bnorm) {
+        S->conv = 1;  // candidate: confirmed against the FP64 residual by the host
+        S->done = 1;
+      } else if (rhon == 0.0 || !isfinite(rhon)) {
+        S->brk = 1;
+        S->done = 1;
+      } else {
+        S->beta = (rhon / S->rho) * (S->alpha / S->omega);
+        S->rho = rhon;
+        if (S->iters >= P.max_iter) S->done = 1;
+      }
+      break;
+    }
+    case PFIN_RESID:
+      // s0 = r . rh (rh = r on init), s1 = r . r
+      S->rr = s1;
+      S->rel = sqrt(s1) / S->bnorm;
+      S->rho = s0;
+      S->conv = (sqrt(s1) <= P.tol * S->bnorm) ? 1 : 0;
+      S->done = S->conv;
+      S->brk = 0;
+      if (init == 1) S->iters = 0;
+      break;
+  }
+}

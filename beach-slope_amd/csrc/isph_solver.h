@@ -1,6 +1,6 @@
-// isph_solver.h -- PPE solve kernels: BiCGStab in the update order of the
-// reference's bicgstab (solver.f90:182-213) with y = A x in the row order of
-// spmv2 (solver.f90:84-127), plus the FP32-inner / FP64-refinement form.
+// isph_solver.h -- PPE solve kernels of the FP64 path: BiCGStab in the update
+// order of the reference's bicgstab (solver.f90:182-213) with y = A x in the
+// row order of spmv2 (solver.f90:84-127).
 //
 // Each BiCGStab iteration is four row kernels, three of which write per-block
 // partial dot products (block_tree over 256 rows); a one-block finalize
@@ -135,123 +135,6 @@ __global__ void __launch_bounds__(256) k_dot(Dev D, const double* a, const doubl
 __global__ void k_copy_sol_to_p(Dev D) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < D.sc->n) D.p[i] = D.sol[i];
-}
-
-// ---- precision form: FP64 residual, FP32 inner BiCGStab, FP64 update ----
-// r = b - A64 x (FP64), r32 = (float) r ; rn2 = r.r (FP64)
-__global__ void __launch_bounds__(256) k_ref_resid(Dev D) {
-  __shared__ double lds[RED_BLOCK];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = D.sc->n;
-  double prod = 0.0;
-  if (i < n) {
-    const double ax = row_spmv<double, double>(D, D.aval, i, D.sol);
-    const double r = D.b[i] - ax;
-    D.r[i] = r;
-    prod = r * r;
-  }
-  const double s = block_tree(prod, lds);
-  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-}
-
-// inner init: d = 0, r32 = rh32 = p32 = (float) r ; rho = rr0 = r32.r32 (FP64)
-__global__ void __launch_bounds__(256) k_in_init(Dev D) {
-  __shared__ double lds[RED_BLOCK];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = D.sc->n;
-  double prod = 0.0;
-  if (i < n) {
-    const float r = (float)D.r[i];
-    D.d32[i] = 0.0f;
-    D.r32[i] = r;
-    D.rh32[i] = r;
-    D.pp32[i] = r;
-    prod = (double)r * (double)r;
-  }
-  const double s = block_tree(prod, lds);
-  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-}
-
-__global__ void __launch_bounds__(256) k_in_1(Dev D) {
-  __shared__ double lds[RED_BLOCK];
-  if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = D.sc->n;
-  double prod = 0.0;
-  if (i < n) {
-    const float t = row_spmv<float, float>(D, D.aval32, i, D.pp32);
-    D.t132[i] = t;
-    prod = (double)t * (double)D.rh32[i];
-  }
-  const double s = block_tree(prod, lds);
-  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
-}
-
-__global__ void __launch_bounds__(256) k_in_2(Dev D) {
-  __shared__ double lds[RED_BLOCK];
-  if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = D.sc->n;
-  const float af = (float)D.sc->alpha;
-  double p1 = 0.0, p2 = 0.0;
-  if (i < n) {
-    const int cap = D.cap, L = D.alen[i];
-    float acc = 0.0f;
-    for (int k = 0; k < L; k++) {
-      const int j = D.acol[k * cap + i];
-      const float sj = D.r32[j] - af * D.t132[j];
-      acc = acc + D.aval32[k * cap + i] * sj;
-    }
-    const float si = D.r32[i] - af * D.t132[i];
-    D.sv32[i] = si;
-    D.t232[i] = acc;
-    p1 = (double)acc * (double)si;
-    p2 = (double)acc * (double)acc;
-  }
-  const double s1 = block_tree(p1, lds);
-  const double s2 = block_tree(p2, lds);
-  if (threadIdx.x == 0) {
-    D.part[blockIdx.x] = s1;
-    D.part[gridDim.x + blockIdx.x] = s2;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_in_3(Dev D) {
-  __shared__ double lds[RED_BLOCK];
-  if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = D.sc->n;
-  const double alpha = D.sc->alpha, omega = D.sc->omega;
-  const float af = (float)alpha, of = (float)omega;
-  double p1 = 0.0, p2 = 0.0;
-  if (i < n) {
-    const float si = D.sv32[i];
-    D.d32[i] = (D.d32[i] + af * D.pp32[i]) + of * si;
-    const float r = si - of * D.t232[i];
-    D.r32[i] = r;
-    p1 = (double)r * (double)D.rh32[i];
-    p2 = (double)r * (double)r;
-  }
-  const double s1 = block_tree(p1, lds);
-  const double s2 = block_tree(p2, lds);
-  if (threadIdx.x == 0) {
-    D.part[blockIdx.x] = s1;
-    D.part[gridDim.x + blockIdx.x] = s2;
-  }
-}
-
-__global__ void k_in_4(Dev D) {
-  if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D.sc->n) return;
-  const float bf = (float)D.sc->beta, of = (float)D.sc->omega;
-  D.pp32[i] = D.r32[i] + bf * (D.pp32[i] - of * D.t132[i]);
-}
-
-// x += (double) d
-__global__ void k_ref_update(Dev D) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < D.sc->n) D.sol[i] = D.sol[i] + (double)D.d32[i];
 }
 
 // ---------------------------------------------------------------------
