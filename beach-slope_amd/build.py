@@ -74,7 +74,7 @@ def build_fortran(verbose: bool = False) -> pathlib.Path | None:
     if not all(s.exists() for s in srcs):
         return None
     cmd = [fc, "-O2", "-module-dir", str(bdir), "-I", str(bdir), *map(str, srcs),
-           "-o", str(exe), f"-L{HERE}", "-lisph_hip", f"-Wl,-rpath,{HERE}"]
+           "-o", str(exe), f"-L{HERE}", "-lisph_hip", "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

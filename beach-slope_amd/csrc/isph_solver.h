@@ -142,28 +142,21 @@ __global__ void k_copy_sol_to_p(Dev D) {
 // preceding row kernel
 // ---------------------------------------------------------------------
 enum {
-  FIN_INIT = 0,      // rho = rr = sum ; convergence test
-  FIN_ALPHA = 1,     // alpha = rho / sum
-  FIN_OMEGA = 2,     // omega = sum0 / sum1
-  FIN_BETA = 3,      // rho' = sum0, rr = sum1 ; iteration bookkeeping, beta
-  FIN_DOT = 4,       // *out = sum
-  FIN_RR = 5,        // rr = sum (refinement residual)
-  FIN_IN_INIT = 6,   // inner: rho = sum ; stop = inner_tol*sqrt(sum)
-  FIN_IN_ALPHA = 7,  // inner: alpha, breakdown before the update
-  FIN_IN_OMEGA = 8,  // inner: omega, breakdown before the update
-  FIN_IN_BETA = 9    // inner: bookkeeping against `stop`
+  FIN_INIT = 0,   // rho = rr = sum ; convergence test
+  FIN_ALPHA = 1,  // alpha = rho / sum
+  FIN_OMEGA = 2,  // omega = sum0 / sum1
+  FIN_BETA = 3,   // rho' = sum0, rr = sum1 ; iteration bookkeeping, beta
+  FIN_DOT = 4     // *out = sum
 };
 
 __global__ void __launch_bounds__(256) k_finalize(Dev D, int op, int nb, double* out) {
   __shared__ double lds[RED_BLOCK];
   DevScalars* S = D.sc;
-  const bool skip = (op == FIN_ALPHA || op == FIN_OMEGA || op == FIN_BETA || op == FIN_IN_ALPHA ||
-                     op == FIN_IN_OMEGA || op == FIN_IN_BETA) && S->done;
+  const bool skip = (op == FIN_ALPHA || op == FIN_OMEGA || op == FIN_BETA) && S->done;
   if (skip) return;
   const double s0 = final_tree(D.part, nb, lds);
   double s1 = 0.0;
-  if (op == FIN_OMEGA || op == FIN_BETA || op == FIN_IN_OMEGA || op == FIN_IN_BETA)
-    s1 = final_tree(D.part + nb, nb, lds);
+  if (op == FIN_OMEGA || op == FIN_BETA) s1 = final_tree(D.part + nb, nb, lds);
   if (threadIdx.x != 0) return;
   const DevParams& P = D.P;
   switch (op) {
@@ -208,47 +201,5 @@ __global__ void __launch_bounds__(256) k_finalize(Dev D, int op, int nb, double*
     case FIN_DOT:
       *out = s0;
       break;
-    case FIN_RR:
-      S->rr = s0;
-      S->rel = sqrt(s0) / S->bnorm;
-      break;
-    case FIN_IN_INIT:
-      S->rho = s0;
-      S->stop = P.inner_tol * sqrt(s0);
-      S->iters = 0;
-      S->brk = 0;
-      S->done = (s0 == 0.0);
-      break;
-    case FIN_IN_ALPHA:
-      S->alpha = S->rho / s0;
-      if (!isfinite(S->alpha)) {
-        S->brk = 1;
-        S->done = 1;
-        S->iters = S->iters + 1;
-      }
-      break;
-    case FIN_IN_OMEGA:
-      S->omega = s0 / s1;
-      if (!isfinite(S->omega) || S->omega == 0.0) {
-        S->brk = 1;
-        S->done = 1;
-        S->iters = S->iters + 1;
-      }
-      break;
-    case FIN_IN_BETA: {
-      const double rhon = s0, rr = s1;
-      S->iters = S->iters + 1;
-      if (sqrt(rr) <= S->stop) {
-        S->done = 1;
-      } else if (rhon == 0.0 || !isfinite(rhon)) {
-        S->brk = 1;
-        S->done = 1;
-      } else {
-        S->beta = (rhon / S->rho) * (S->alpha / S->omega);
-        S->rho = rhon;
-        if (S->iters >= P.max_iter) S->done = 1;
-      }
-      break;
-    }
   }
 }

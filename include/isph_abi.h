@@ -55,7 +55,15 @@ enum { ISPH_F_BUFFER = 1, ISPH_F_FREE = 2, ISPH_F_VICINITY = 4 };
  * integrator.f90:943-949); a larger count is reported as ISPH_ENONFINITE. */
 #define ISPH_MAX_SOLSTEPS 100000
 
-/* PPE precision modes. */
+/* PPE solve forms.
+ *   ISPH_PPE_FP64        BiCGStab in FP64, iterates identical to the CPU
+ *                        restatement (the parity path).
+ *   ISPH_PPE_FP32_REFINE BiCGStab right-preconditioned by a cell-aggregation
+ *                        multigrid V-cycle; FP32 matrix, V-cycle and search
+ *                        directions, FP64 solution, residual and dot products;
+ *                        a convergence claim is checked against the FP64
+ *                        residual b - A x and the iteration restarts from it
+ *                        when the check fails (iterative refinement). */
 enum { ISPH_PPE_FP64 = 0, ISPH_PPE_FP32_REFINE = 1 };
 
 /*
@@ -68,7 +76,7 @@ typedef struct isph_params {
   int32_t fixed_dt;       /* 1: dt stays fixed; 0: adaptive (integrator.f90:927-937) */
   int32_t precision;      /* ISPH_PPE_FP64 or ISPH_PPE_FP32_REFINE             */
   int32_t max_iter;       /* BiCGStab iteration cap per solve                  */
-  int32_t max_outer;      /* refinement sweeps cap (precision form)            */
+  int32_t max_outer;      /* restarts cap of the precision form                */
   double h;               /* smoothing length h1 (setup.f90:47)                */
   double dl;              /* lattice spacing dl (setup.f90:53)                 */
   double lam;             /* 1e-3*h^2 (setup.f90:82)                           */
@@ -91,13 +99,13 @@ typedef struct isph_params {
   double buf_xmax;
   double dt;              /* initial (or fixed) time step                      */
   double tol;             /* PPE relative residual target ||b-Ax||/||b||       */
-  double inner_tol;       /* FP32 inner solve relative tolerance (precision form) */
+  double inner_tol;       /* reserved, unused (kept for layout stability)      */
 } isph_params;
 
 /* Statistics of the last PPE solve. */
 typedef struct isph_solve_stats {
-  int32_t iterations;     /* BiCGStab iterations (inner, summed over sweeps)   */
-  int32_t outer;          /* refinement sweeps (0 for the FP64 path)           */
+  int32_t iterations;     /* BiCGStab iterations (summed over restarts)        */
+  int32_t outer;          /* restarts of the precision form (0 for FP64 path)  */
   int32_t converged;      /* 1 if the target was met                           */
   int32_t breakdown;      /* 1 if rho or omega became zero / non-finite        */
   double rel_residual;    /* final ||b-Ax||/||b|| (recursive for FP64 path)    */
@@ -189,6 +197,12 @@ int isph_dump_ppe(isph_ctx *ctx, int32_t *row_len, int32_t *col, double *val,
  * (for solver tests on the assembled matrix). */
 int isph_ppe_set_rhs(isph_ctx *ctx, const double *rhs, const double *x0);
 int isph_ppe_get_solution(isph_ctx *ctx, double *x);
+/* Load a whole PPE system (n rows in the canonical row-major layout of
+ * isph_dump_ppe, n <= capacity) in place of the assembled one, for solver
+ * tests on systems assembled elsewhere.  Columns must lie in [0, n). */
+int isph_load_system(isph_ctx *ctx, int32_t n, const int32_t *row_len,
+                     const int32_t *col, const double *val, const double *rhs,
+                     const double *x0);
 
 /* ---- batched form: many independent scenarios in one context set ---- */
 typedef struct isph_batch isph_batch;
