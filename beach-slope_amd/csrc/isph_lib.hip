@@ -3,6 +3,7 @@
 // stream.  Every entry point maps to one reference subroutine of the
 // production loop (main.f90:199-261) or to a fused sequence of them.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -480,39 +481,51 @@ static std::pair<hipEvent_t, hipEvent_t> timer_pair(KTimer& t) {
   return p;
 }
 
-// collect the elapsed times of completed launches (called after a sync)
-static void timers_collect(isph_ctx* c) {
+// Collect the elapsed times of completed launches (called after a sync).
+// Only the first `keep` pending launches of each timer are counted: the
+// solver launches whole chunks of iterations, and the launches after the
+// device has set its done flag return at once; they are dropped so that the
+// average is over launches that did the full work.  keep < 0 counts all.
+static void timers_collect(isph_ctx* c, int64_t keep) {
   for (KTimer& t : c->timers) {
+    int64_t k = 0;
     for (auto& pr : t.pending) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+      if ((keep < 0 || k < keep) && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
         t.total_ms += ms;
         t.launches += 1;
       }
+      k++;
       t.pool.push_back(pr);
     }
     t.pending.clear();
   }
 }
 
+// A timed launch: the two events are attached to the kernel's own dispatch
+// (hipExtLaunchKernelGGL), so their difference is the kernel's execution
+// time without the gaps around it.
 #define TIMED_LAUNCH(tidx, kern, nthreads, ...)                                      \
   do {                                                                               \
-    std::pair<hipEvent_t, hipEvent_t> ev_{};                                         \
     if (c->prof) {                                                                   \
-      ev_ = timer_pair(c->timers[tidx]);                                             \
-      (void)hipEventRecord(ev_.first, c->stream);                                    \
-    }                                                                                \
-    LAUNCH(kern, nthreads, __VA_ARGS__);                                             \
-    if (c->prof) {                                                                   \
-      (void)hipEventRecord(ev_.second, c->stream);                                   \
+      std::pair<hipEvent_t, hipEvent_t> ev_ = timer_pair(c->timers[tidx]);           \
+      hipExtLaunchKernelGGL(kern, dim3(grid_of(nthreads)), dim3(256), 0, c->stream,  \
+                            ev_.first, ev_.second, 0, __VA_ARGS__);                  \
+      hipError_t e_ = hipGetLastError();                                             \
+      if (e_ != hipSuccess) {                                                        \
+        set_err(c, "launch of %s failed: %s", #kern, hipGetErrorString(e_));        \
+        return ISPH_EHIP;                                                            \
+      }                                                                              \
       c->timers[tidx].pending.push_back(ev_);                                        \
+    } else {                                                                         \
+      LAUNCH(kern, nthreads, __VA_ARGS__);                                           \
     }                                                                                \
   } while (0)
 
 extern "C" int isph_prof_enable(isph_ctx* c, int32_t on) {
   if (!c) return ISPH_EARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  timers_collect(c);
+  timers_collect(c, -1);
   c->prof = on != 0;
   for (KTimer& t : c->timers) {
     t.total_ms = 0;
@@ -525,7 +538,7 @@ extern "C" int isph_prof_read(isph_ctx* c, const char* kernel, double* total_ms,
                               int64_t* launches) {
   if (!c || !kernel) return ISPH_EARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  timers_collect(c);
+  timers_collect(c, -1);
   for (KTimer& t : c->timers) {
     if (std::strcmp(t.name, kernel) == 0) {
       if (total_ms) *total_ms = t.total_ms;
@@ -723,6 +736,7 @@ static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
   rc = read_scalars(c, &s);
   if (rc) return rc;
   int restarts = 0, chunk = 4;
+  int it0 = s.iters;  // iterations before this restart cycle
   while (!s.conv) {
     for (int k = 0; k < chunk; k++) {
       rc = mg_apply<float>(c, c->D.pp32, ph);
@@ -740,11 +754,11 @@ static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
     }
     rc = read_scalars(c, &s);
     if (rc) return rc;
-    if (c->prof) timers_collect(c);
     if (!s.done) {
       if (chunk < 16) chunk *= 2;
       continue;
     }
+    if (c->prof) timers_collect(c, s.iters - it0);
     // the FP64 residual decides; restart from it unless it is met
     hipLaunchKernelGGL(k_pb_resid, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, 2);
     PFINALIZE(PFIN_RESID, n, 2);
@@ -753,6 +767,7 @@ static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
     if (s.conv) break;
     if (s.iters >= c->P.max_iter || ++restarts > c->P.max_outer) break;
     chunk = 4;
+    it0 = s.iters;
   }
   st->iterations = s.iters;
   st->outer = restarts;
@@ -781,11 +796,11 @@ static int bicg_loop(isph_ctx* c, int* iters_out, int* brk_out, int* conv_out,
     it += chunk;
     int rc = read_scalars(c, &s);
     if (rc) return rc;
-    if (c->prof) timers_collect(c);
     if (s.done) break;
     if (chunk < 32) chunk *= 2;
     if (it > c->P.max_iter + 64) break;
   }
+  if (c->prof) timers_collect(c, s.iters);
   *iters_out = s.iters;
   *brk_out = s.brk;
   *conv_out = s.conv;

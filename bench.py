@@ -10,22 +10,31 @@ device resident before the timed region starts.
 
 Two forms are measured on the same input:
   plain   FP64 BiCGStab (the parity path);
-  chosen  FP32 BiCGStab inner solves + FP64 residual and iterative refinement.
+  chosen  BiCGStab right-preconditioned by a cell-multigrid V-cycle, FP32
+          matrix, V-cycle and directions, FP64 solution, residual and dots,
+          with a final FP64 residual check.
 Both solve to the same relative residual (1e-8) so their throughputs compare
 like for like.  `value` is the chosen form.
 
 The roofline object is for the dominant kernel, the solver's SpMV t = A s
 with its two fused dot products, timed with HIP events on the context's
-stream over the timed region.  The CPU baseline is the oracle (a C/OpenMP
-restatement of the reference) on a bounded sample of the same workload.
+stream over the timed region.  Its `traffic` (HBM bytes per launch) comes
+from the FETCH_SIZE / WRITE_SIZE counter passes summarised in
+profiles/traffic_flume_650k.md, when that file is present.  The CPU baseline
+is the oracle (a C/OpenMP restatement of the reference) on a bounded sample
+of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode both|plain|chosen]
+       python bench.py --copy-calibration   (a 1 GiB device copy, for the
+                                             counter calibration only)
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import pathlib
+import re
 import sys
 import time
 
@@ -68,7 +77,7 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
     n = g.n
     g.prof_enable(False)
     g.close()
-    wall = t1 - t1 + (t1 - t0)
+    wall = t1 - t0
     avg_us = 1e3 * ms / max(launches, 1)
     bytes_per = spmv_s_bytes(n, nnz, bool(precision))
     achieved = bytes_per / (avg_us * 1e-6) / 1e9 if launches else 0.0
@@ -79,6 +88,37 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
         kernel=kname, kernel_avg_us=avg_us, kernel_launches=launches,
         kernel_bytes=bytes_per, kernel_gbs=achieved,
     )
+
+
+KERNEL_OF = {"spmv_s": "k_bicg_2", "spmv_s32": "k_pb_2"}
+TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_flume_650k.md"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the counter summary, or None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        m = re.match(r"^traffic\s+(\S+)\s+([0-9.eE+]+)\s*$", line.strip())
+        if m and m.group(1) == KERNEL_OF.get(kernel, kernel):
+            return float(m.group(2))
+    return None
+
+
+def copy_calibration() -> int:
+    """Three launches of a plain 1 GiB device-to-device copy (torch's copy
+    kernel): the FETCH_SIZE and WRITE_SIZE counters of one launch should read
+    1 GiB each; the ratio calibrates the counters of the solver kernels."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
+    return 0
 
 
 def cpu_baseline(oracle, case, tol: float):
@@ -103,10 +143,13 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["both", "plain", "chosen"])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(independent scenarios would shard one per GPU, see the design notes)",
+              "(one flume runs on one GPU; independent scenarios would be placed one per GPU)",
               file=sys.stderr)
         return 2
     pkg = entry.load_package()
@@ -145,7 +188,7 @@ def main() -> int:
                    "form": "chosen" if "chosen" in res else "plain"},
         "roofline": {"bound": "hbm", "achieved": head["kernel_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["kernel_gbs"] / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": head["kernel"],
+                     "traffic": measured_traffic(head["kernel"]), "kernel": head["kernel"],
                      "avg_launch_us": head["kernel_avg_us"]},
         "cpu_baseline": cpu,
     }
