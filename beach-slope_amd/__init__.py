@@ -171,9 +171,10 @@ class Scenario:
     # -- plumbing --------------------------------------------------------
     def _check(self, rc: int, where: str, ctx: bool = True):
         if rc != 0:
-            msg = ""
-            if ctx and self._ctx:
-                msg = (lib().isph_last_error(self._ctx) or b"").decode()
+            # without a context (failed isph_create) the library keeps the
+            # message of the calling thread's last create
+            handle = self._ctx if (ctx and self._ctx) else None
+            msg = (lib().isph_last_error(handle) or b"").decode()
             raise IsphError(rc, where, msg)
 
     def close(self):

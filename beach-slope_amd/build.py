@@ -29,7 +29,8 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-fPIC", "-shared", "-Wno-unused-result",
+    "-fPIC", "-shared", "-Wno-unused-result", "-Werror=infinite-recursion",
+    "-Werror=return-type", "-Werror=uninitialized",
 ]
 
 

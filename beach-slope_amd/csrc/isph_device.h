@@ -57,6 +57,7 @@ struct Dev {
   DevParams P;
   DevScalars* sc;
   int cap;
+  int nblk;  // row blocks of RED_BLOCK rows covering the host's upper bound of n
   // gathered particle state (ping-pong buffers swapped by the host)
   double *x, *y, *u, *v, *us, *vs, *xs, *ys, *p, *m, *rho, *rho0, *vol0, *c;
   uint8_t *tid, *flags, *vic;
@@ -79,6 +80,51 @@ struct Dev {
   double *r, *rh, *pp, *t1, *sv, *t2, *part;
   float *r32, *rh32, *pp32, *t132, *sv32, *t232, *d32;
 };
+
+// ---------------------------------------------------------------------
+// Row-block mapping of the solver's row kernels.  Workgroups are dispatched
+// to the 8 XCDs round robin (workgroup b runs on XCD b mod 8) and each XCD
+// has its own L2.  The grid is launched as 8 * per workgroups, and workgroup
+// b takes row block (b mod 8) * per + b / 8, so every XCD sweeps one
+// contiguous eighth of the rows: the neighbour gathers of a row then hit the
+// L2 of the XCD that loaded the neighbouring rows.  Workgroups whose block is
+// at or beyond nblk return at once (all their threads together).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int row_block() {
+  const int per = gridDim.x >> 3;
+  const int b = blockIdx.x;
+  return (b & 7) * per + (b >> 3);
+}
+
+// Sum over the ELL row i (L entries, slot k at k*cap + i) of val * gather(col),
+// U entries at a time: the column and value loads of U entries are issued
+// together, then their U gathers, then the products are added in entry
+// order.  The result is the sequential sum acc = acc + val_k * x_k, rounded
+// exactly as the one-entry-at-a-time loop.
+template <int U, typename A, typename VAL, typename G>
+__device__ __forceinline__ A ell_row_sum(const int32_t* __restrict__ col,
+                                         const VAL* __restrict__ val, int cap, int i, int L,
+                                         G gather) {
+  A acc = (A)0;
+  for (int k0 = 0; k0 < L; k0 += U) {
+    int c[U];
+    VAL v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool in = k0 + u < L;
+      c[u] = in ? col[(k0 + u) * cap + i] : i;
+      v[u] = in ? val[(k0 + u) * cap + i] : (VAL)0;
+    }
+    A g[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) g[u] = (k0 + u < L) ? gather(c[u]) : (A)0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u < L) acc = acc + v[u] * g[u];
+  }
+  return acc;
+}
+#define ELL_U 8
 
 // ---------------------------------------------------------------------
 // kernel.f90 restatement (device)

@@ -67,7 +67,28 @@ static void set_err(isph_ctx* c, const char* fmt, ...) {
   c->err = buf;
 }
 
+// message of the calling thread's last failed isph_create (no context exists
+// to hold it); returned by isph_last_error(NULL)
+static thread_local std::string g_create_err;
+
+static int create_fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_create_err = buf;
+  return code;
+}
+
 static int grid_of(int n) { return std::max(1, (n + 255) / 256); }
+// grid of a row kernel mapped by row_block(): a multiple of the 8 XCDs
+static int row_grid(int n) { return 8 * ((grid_of(n) + 7) / 8); }
+// the host's upper bound of the live particle count and the row blocks over it
+static void set_nhi(isph_ctx* c, int n) {
+  c->n_hi = n;
+  c->D.nblk = grid_of(n);
+}
 
 template <typename T>
 static int dalloc(isph_ctx* c, T** p, int64_t count) {
@@ -127,19 +148,27 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
 extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_solid_cap,
                            isph_ctx** out) {
   // slot-major ELL indices k*cap + i are int32: cap*ROW_CAP must stay below 2^31
-  if (!params || !out || n_cap <= 0 || n_cap * ROW_CAP >= (1ll << 31) || n_solid_cap < 0)
-    return ISPH_EARG;
-  if (params->cellx < 3 || params->celly < 3 || !(params->h > 0)) return ISPH_EARG;
-  if ((int64_t)params->cellx * params->celly * NCLS + 1 > (1ll << 30)) return ISPH_EARG;
+  if (out) *out = nullptr;
+  g_create_err.clear();
+  if (!params || !out) return create_fail(ISPH_EARG, "isph_create: null argument");
+  if (n_cap <= 0 || n_cap * ROW_CAP >= (1ll << 31) || n_solid_cap < 0)
+    return create_fail(ISPH_EARG, "isph_create: capacity %lld outside (0, 2^31/%d)",
+                       (long long)n_cap, ROW_CAP);
+  if (params->cellx < 3 || params->celly < 3 || !(params->h > 0))
+    return create_fail(ISPH_EARG, "isph_create: cell grid %d x %d or h %g invalid",
+                       params->cellx, params->celly, params->h);
+  if ((int64_t)params->cellx * params->celly * NCLS + 1 > (1ll << 30))
+    return create_fail(ISPH_EARG, "isph_create: cell grid too large");
   isph_ctx* c = new isph_ctx();
   c->P = *params;
   c->cap = n_cap;
   c->scap = std::max<int64_t>(n_solid_cap, 1);
   c->ncell = params->cellx * params->celly;
-  hipGetDevice(&c->device);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  hipError_t he = hipGetDevice(&c->device);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
     delete c;
-    return ISPH_EHIP;
+    return create_fail(ISPH_EHIP, "isph_create: no usable HIP device: %s", hipGetErrorString(he));
   }
   Dev& D = c->D;
   D.cap = (int)n_cap;
@@ -205,9 +234,9 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
     G.cscale = 1.6f;
   }
   if (rc) {
-    std::string e = c->err;
+    const std::string e = c->err;
     isph_destroy(c);
-    return ISPH_EHIP;
+    return create_fail(ISPH_EHIP, "isph_create: %s", e.c_str());
   }
   c->G.done = &D.sc->done;
   if (c->G.nlev >= 1)
@@ -216,9 +245,10 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   s.dt = params->dt;
   s.dtsol = params->dt;
   s.solsteps = 1;
-  if (hipMemcpy(D.sc, &s, sizeof s, hipMemcpyHostToDevice) != hipSuccess) {
+  he = hipMemcpy(D.sc, &s, sizeof s, hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
     isph_destroy(c);
-    return ISPH_EHIP;
+    return create_fail(ISPH_EHIP, "isph_create: hipMemcpy failed: %s", hipGetErrorString(he));
   }
   hipMemset(D.sstart, 0, sizeof(int32_t) * (c->ncell + 1));
   hipMemset(D.vic, 0, N);
@@ -240,7 +270,8 @@ extern "C" int isph_destroy(isph_ctx* c) {
 }
 
 extern "C" const char* isph_last_error(const isph_ctx* c) {
-  return c ? c->err.c_str() : "null context";
+  if (c) return c->err.c_str();
+  return g_create_err.empty() ? "null context" : g_create_err.c_str();
 }
 
 extern "C" int isph_set_params(isph_ctx* c, const isph_params* p) {
@@ -363,7 +394,7 @@ extern "C" int isph_upload(isph_ctx* c, int64_t n, const double* x, const double
   s.umax_bits = 0;
   s.numax_bits = 0;
   HIPCHK(c, hipMemcpy(D.sc, &s, sizeof s, hipMemcpyHostToDevice));
-  c->n_hi = (int)n;
+  set_nhi(c, (int)n);
   c->assembled = false;
   c->have_nbr = false;
   return 0;
@@ -413,7 +444,7 @@ extern "C" int isph_download(isph_ctx* c, int64_t* n_out, double* x, double* y, 
   int rc = read_scalars(c, &s);
   if (rc) return rc;
   const int64_t n = s.n;
-  c->n_hi = s.n;
+  set_nhi(c, s.n);
   if (n_out) *n_out = n;
   Dev& D = c->D;
   struct { double* dst; double* src; } f[] = {{x, D.x}, {y, D.y}, {u, D.u}, {v, D.v}, {p, D.p},
@@ -469,6 +500,17 @@ extern "C" int isph_set_scalars(isph_ctx* c, const isph_scalars* in) {
     }                                                                                \
   } while (0)
 
+// launch of a row kernel mapped by row_block() (nthreads = c->n_hi)
+#define RLAUNCH(kern, nthreads, ...)                                                 \
+  do {                                                                               \
+    hipLaunchKernelGGL(kern, dim3(row_grid(nthreads)), dim3(256), 0, c->stream, __VA_ARGS__); \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess) {                                                          \
+      set_err(c, "launch of %s failed: %s", #kern, hipGetErrorString(e_));          \
+      return ISPH_EHIP;                                                              \
+    }                                                                                \
+  } while (0)
+
 static std::pair<hipEvent_t, hipEvent_t> timer_pair(KTimer& t) {
   if (!t.pool.empty()) {
     auto p = t.pool.back();
@@ -502,14 +544,14 @@ static void timers_collect(isph_ctx* c, int64_t keep) {
   }
 }
 
-// A timed launch: the two events are attached to the kernel's own dispatch
+// A timed launch of a row kernel: the two events are attached to the kernel's own dispatch
 // (hipExtLaunchKernelGGL), so their difference is the kernel's execution
 // time without the gaps around it.
 #define TIMED_LAUNCH(tidx, kern, nthreads, ...)                                      \
   do {                                                                               \
     if (c->prof) {                                                                   \
       std::pair<hipEvent_t, hipEvent_t> ev_ = timer_pair(c->timers[tidx]);           \
-      hipExtLaunchKernelGGL(kern, dim3(grid_of(nthreads)), dim3(256), 0, c->stream,  \
+      hipExtLaunchKernelGGL(kern, dim3(row_grid(nthreads)), dim3(256), 0, c->stream, \
                             ev_.first, ev_.second, 0, __VA_ARGS__);                  \
       hipError_t e_ = hipGetLastError();                                             \
       if (e_ != hipSuccess) {                                                        \
@@ -518,7 +560,7 @@ static void timers_collect(isph_ctx* c, int64_t keep) {
       }                                                                              \
       c->timers[tidx].pending.push_back(ev_);                                        \
     } else {                                                                         \
-      LAUNCH(kern, nthreads, __VA_ARGS__);                                           \
+      RLAUNCH(kern, nthreads, __VA_ARGS__);                                          \
     }                                                                                \
   } while (0)
 
@@ -685,7 +727,7 @@ template <typename VT>
 static int mg_apply(isph_ctx* c, const VT* v, float* out) {
   const int n = c->n_hi;
   MgDev& G = c->G;
-  hipLaunchKernelGGL(k_mg_pre0<VT>, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, G, v,
+  hipLaunchKernelGGL(k_mg_pre0<VT>, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, G, v,
                      c->D.d32);
   hipLaunchKernelGGL(k_mg_restrict0, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, c->D, G,
                      (const float*)c->D.d32);
@@ -696,7 +738,7 @@ static int mg_apply(isph_ctx* c, const VT* v, float* out) {
   } else {
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_CO_THREADS), 0, c->stream, G, 1);
   }
-  hipLaunchKernelGGL(k_mg_post0<VT>, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, G, v, out);
+  hipLaunchKernelGGL(k_mg_post0<VT>, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, G, v, out);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -730,7 +772,7 @@ static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
   if (rc) return rc;
   float* ph = c->D.r32;
   float* sh = c->D.sv32;
-  hipLaunchKernelGGL(k_pb_resid, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, 1);
+  hipLaunchKernelGGL(k_pb_resid, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, 1);
   PFINALIZE(PFIN_RESID, n, 1);
   DevScalars s;
   rc = read_scalars(c, &s);
@@ -743,14 +785,14 @@ static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
       if (rc) return rc;
       TIMED_LAUNCH(2, k_pb_1, n, c->D, c->G, (const float*)ph);
       PFINALIZE(PFIN_ALPHA, n, 0);
-      LAUNCH(k_pb_s, n, c->D);
+      RLAUNCH(k_pb_s, n, c->D);
       rc = mg_apply<double>(c, c->D.sv, sh);
       if (rc) return rc;
       TIMED_LAUNCH(3, k_pb_2, n, c->D, c->G, (const float*)sh);
       PFINALIZE(PFIN_OMEGA, n, 0);
-      LAUNCH(k_pb_3, n, c->D, (const float*)ph, (const float*)sh);
+      RLAUNCH(k_pb_3, n, c->D, (const float*)ph, (const float*)sh);
       PFINALIZE(PFIN_BETA, n, 0);
-      LAUNCH(k_pb_4, n, c->D);
+      RLAUNCH(k_pb_4, n, c->D);
     }
     rc = read_scalars(c, &s);
     if (rc) return rc;
@@ -760,7 +802,7 @@ static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
     }
     if (c->prof) timers_collect(c, s.iters - it0);
     // the FP64 residual decides; restart from it unless it is met
-    hipLaunchKernelGGL(k_pb_resid, dim3(grid_of(n)), dim3(256), 0, c->stream, c->D, 2);
+    hipLaunchKernelGGL(k_pb_resid, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, 2);
     PFINALIZE(PFIN_RESID, n, 2);
     rc = read_scalars(c, &s);
     if (rc) return rc;
@@ -789,9 +831,9 @@ static int bicg_loop(isph_ctx* c, int* iters_out, int* brk_out, int* conv_out,
       FINALIZE(FIN_ALPHA, n);
       TIMED_LAUNCH(1, k_bicg_2, n, c->D);
       FINALIZE(FIN_OMEGA, n);
-      LAUNCH(k_bicg_3, n, c->D);
+      RLAUNCH(k_bicg_3, n, c->D);
       FINALIZE(FIN_BETA, n);
-      LAUNCH(k_bicg_4, n, c->D);
+      RLAUNCH(k_bicg_4, n, c->D);
     }
     it += chunk;
     int rc = read_scalars(c, &s);
@@ -822,7 +864,7 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
     st.nnz = (double)nnz;
   }
   // ||b||
-  LAUNCH(k_dot, n, c->D, c->D.b, c->D.b, c->d_bn2);
+  RLAUNCH(k_dot, n, c->D, c->D.b, c->D.b);
   FINALIZE_OUT(FIN_DOT, n, c->d_bn2);
   double bn2 = 0;
   HIPCHK(c, hipMemcpyAsync(&bn2, c->d_bn2, 8, hipMemcpyDeviceToHost, c->stream));
@@ -840,7 +882,7 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
       st.converged = 1;
     }
   } else if (c->P.precision == ISPH_PPE_FP64) {
-    LAUNCH(k_bicg_init, n, c->D);
+    RLAUNCH(k_bicg_init, n, c->D);
     FINALIZE(FIN_INIT, n);
     rc = read_scalars(c, &s);
     if (rc) return rc;
@@ -966,7 +1008,7 @@ static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
   DevScalars s;
   int rc = read_scalars(c, &s);  // solsteps from the previous timestep
   if (rc) return rc;
-  c->n_hi = s.n;
+  set_nhi(c, s.n);
   if (s.solsteps < 1 || s.solsteps > ISPH_MAX_SOLSTEPS) {
     set_err(c, "scalar sub-step count %d outside [1, %d]", s.solsteps, ISPH_MAX_SOLSTEPS);
     return ISPH_ENONFINITE;
@@ -1149,7 +1191,7 @@ extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, cons
   if (rc) return rc;
   s.n = n;
   HIPCHK(c, hipMemcpy(c->D.sc, &s, sizeof s, hipMemcpyHostToDevice));
-  c->n_hi = n;
+  set_nhi(c, n);
   c->assembled = true;
   return 0;
 }

@@ -185,15 +185,14 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgDev G, double* work) {
 template <typename VT>
 __global__ void __launch_bounds__(256) k_mg_pre0(Dev D, MgDev G, const VT* __restrict__ v,
                                                  float* __restrict__ r0) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   if (*G.done || i >= D.sc->n) return;
-  const int cap = D.cap, L = D.alen[i];
   const float w = G.w;
-  float acc = 0.0f;
-  for (int k = 0; k < L; k++) {
-    const int j = D.acol[k * cap + i];
-    acc = acc + D.aval32[k * cap + i] * (w * (float)v[j] * G.d0[j]);
-  }
+  const float* __restrict__ d0 = G.d0;
+  const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
+                                              [&](int j) { return w * (float)v[j] * d0[j]; });
   const float vi = (float)v[i];
   G.x0[i] = w * vi * G.d0[i];
   r0[i] = vi - acc;
@@ -214,16 +213,16 @@ __global__ void k_mg_restrict0(Dev D, MgDev G, const float* __restrict__ r0) {
 template <typename VT>
 __global__ void __launch_bounds__(256) k_mg_post0(Dev D, MgDev G, const VT* __restrict__ v,
                                                   float* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   if (*G.done || i >= D.sc->n) return;
-  const int cap = D.cap, L = D.alen[i];
   const float c = G.cscale;
-  const float* e1 = G.e1;
-  float acc = 0.0f;
-  for (int k = 0; k < L; k++) {
-    const int j = D.acol[k * cap + i];
-    acc = acc + D.aval32[k * cap + i] * (G.x0[j] + c * e1[D.cell[j]]);
-  }
+  const float* __restrict__ e1 = G.e1;
+  const float* __restrict__ x0 = G.x0;
+  const int32_t* __restrict__ cell = D.cell;
+  const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
+                                              [&](int j) { return x0[j] + c * e1[cell[j]]; });
   const float x1 = G.x0[i] + c * e1[D.cell[i]];
   out[i] = x1 + G.w * ((float)v[i] - acc) * G.d0[i];
 }
@@ -364,24 +363,27 @@ __global__ void k_mg_l1_up(MgDev G) {
 __global__ void __launch_bounds__(256) k_pb_1(Dev D, MgDev G, const float* __restrict__ ph) {
   __shared__ double lds[RED_BLOCK];
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   double prod = 0.0;
   if (i < n) {
-    const int cap = D.cap, L = D.alen[i];
-    float acc = 0.0f;
-    for (int k = 0; k < L; k++) acc = acc + D.aval32[k * cap + i] * ph[D.acol[k * cap + i]];
+    const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
+                                                [&](int j) { return ph[j]; });
     D.t132[i] = acc;
     prod = (double)acc * (double)D.rh32[i];
   }
   const double s = block_tree(prod, lds);
-  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (threadIdx.x == 0) D.part[vb] = s;
 }
 
 // s = r - alpha t1 (FP64 and an FP32 copy for the V-cycle)
 __global__ void k_pb_s(Dev D) {
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
   const double s = D.r[i] - D.sc->alpha * (double)D.t132[i];
   D.sv[i] = s;
@@ -391,13 +393,14 @@ __global__ void k_pb_s(Dev D) {
 __global__ void __launch_bounds__(256) k_pb_2(Dev D, MgDev G, const float* __restrict__ sh) {
   __shared__ double lds[RED_BLOCK];
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
-    const int cap = D.cap, L = D.alen[i];
-    float acc = 0.0f;
-    for (int k = 0; k < L; k++) acc = acc + D.aval32[k * cap + i] * sh[D.acol[k * cap + i]];
+    const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
+                                                [&](int j) { return sh[j]; });
     D.t232[i] = acc;
     p1 = (double)acc * D.sv[i];
     p2 = (double)acc * (double)acc;
@@ -405,8 +408,8 @@ __global__ void __launch_bounds__(256) k_pb_2(Dev D, MgDev G, const float* __res
   const double s1 = block_tree(p1, lds);
   const double s2 = block_tree(p2, lds);
   if (threadIdx.x == 0) {
-    D.part[blockIdx.x] = s1;
-    D.part[gridDim.x + blockIdx.x] = s2;
+    D.part[vb] = s1;
+    D.part[D.nblk + vb] = s2;
   }
 }
 
@@ -415,7 +418,9 @@ __global__ void __launch_bounds__(256) k_pb_3(Dev D, const float* __restrict__ p
                                               const float* __restrict__ sh) {
   __shared__ double lds[RED_BLOCK];
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   const double alpha = D.sc->alpha, omega = D.sc->omega;
   double p1 = 0.0, p2 = 0.0;
@@ -429,15 +434,17 @@ __global__ void __launch_bounds__(256) k_pb_3(Dev D, const float* __restrict__ p
   const double s1 = block_tree(p1, lds);
   const double s2 = block_tree(p2, lds);
   if (threadIdx.x == 0) {
-    D.part[blockIdx.x] = s1;
-    D.part[gridDim.x + blockIdx.x] = s2;
+    D.part[vb] = s1;
+    D.part[D.nblk + vb] = s2;
   }
 }
 
 // p = r + beta (p - omega t1)  (p stored FP32)
 __global__ void k_pb_4(Dev D) {
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
   const double beta = D.sc->beta, omega = D.sc->omega;
   D.pp32[i] = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * (double)D.t132[i]));
@@ -446,13 +453,15 @@ __global__ void k_pb_4(Dev D) {
 // r = b - A64 x (FP64 true residual); on init also rh32 = p32 = (float) r
 __global__ void __launch_bounds__(256) k_pb_resid(Dev D, int init) {
   __shared__ double lds[RED_BLOCK];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
-    const int cap = D.cap, L = D.alen[i];
-    double acc = 0.0;
-    for (int k = 0; k < L; k++) acc = acc + D.aval[k * cap + i] * D.sol[D.acol[k * cap + i]];
+    const double* __restrict__ sol = D.sol;
+    const double acc = ell_row_sum<ELL_U, double>(D.acol, D.aval, D.cap, i, D.alen[i],
+                                                  [&](int j) { return sol[j]; });
     const double r = D.b[i] - acc;
     D.r[i] = r;
     if (init) {
@@ -465,8 +474,8 @@ __global__ void __launch_bounds__(256) k_pb_resid(Dev D, int init) {
   const double s1 = block_tree(p1, lds);
   const double s2 = block_tree(p2, lds);
   if (threadIdx.x == 0) {
-    D.part[blockIdx.x] = s1;
-    D.part[gridDim.x + blockIdx.x] = s2;
+    D.part[vb] = s1;
+    D.part[D.nblk + vb] = s2;
   }
 }
 

@@ -3,33 +3,36 @@
 // row order of spmv2 (solver.f90:84-127).
 //
 // Each BiCGStab iteration is four row kernels, three of which write per-block
-// partial dot products (block_tree over 256 rows); a one-block finalize
-// kernel then sums the partials in the fixed order (final_tree) and forms
-// alpha, omega or beta on the device.  The kernel boundary makes the
+// partial dot products (block_tree over 256 rows, in row-block order whatever
+// workgroup computes the block); a one-block finalize kernel then sums the
+// partials in the fixed order (final_tree) and forms alpha, omega or beta on
+// the device.  The kernel boundary makes the
 // partials visible, so no device-scope fence is needed inside the row
 // kernels.  The host only polls the `done` flag every few iterations.
 #pragma once
 #include "isph_device.h"
 
 // y_i = sum_k A_ik x_col(k), sequential in row order (spmv2)
-template <typename T, typename V>
-__device__ __forceinline__ V row_spmv(const Dev& D, const T* __restrict__ val, int i,
-                                      const V* __restrict__ x) {
-  const int cap = D.cap, L = D.alen[i];
-  V acc = (V)0;
-  for (int k = 0; k < L; k++) acc = acc + (V)val[k * cap + i] * x[D.acol[k * cap + i]];
-  return acc;
+__device__ __forceinline__ double row_spmv(const Dev& D, const double* __restrict__ val, int i,
+                                           const double* __restrict__ x) {
+  return ell_row_sum<ELL_U, double>(D.acol, val, D.cap, i, D.alen[i],
+                                    [&](int j) { return x[j]; });
 }
 
 // ---- FP64 path ----
-// r = b - A x ; rh = r ; p = r ; partial (r.r) ; last block: rho = rr
+// Row kernels run on the XCD-contiguous row blocks of row_block(); block vb
+// writes partial vb (and nblk + vb for a second dot product).
+
+// r = b - A x ; rh = r ; p = r ; partial (r.r)
 __global__ void __launch_bounds__(256) k_bicg_init(Dev D) {
   __shared__ double lds[RED_BLOCK];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   double prod = 0.0;
   if (i < n) {
-    const double ax = row_spmv<double, double>(D, D.aval, i, D.sol);
+    const double ax = row_spmv(D, D.aval, i, D.sol);
     const double r = D.b[i] - ax;
     D.r[i] = r;
     D.rh[i] = r;
@@ -37,23 +40,25 @@ __global__ void __launch_bounds__(256) k_bicg_init(Dev D) {
     prod = r * r;
   }
   const double s = block_tree(prod, lds);
-  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (threadIdx.x == 0) D.part[vb] = s;
 }
 
 // t1 = A p ; alpha = rho / (t1 . rh)
 __global__ void __launch_bounds__(256) k_bicg_1(Dev D) {
   __shared__ double lds[RED_BLOCK];
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   double prod = 0.0;
   if (i < n) {
-    const double t = row_spmv<double, double>(D, D.aval, i, D.pp);
+    const double t = row_spmv(D, D.aval, i, D.pp);
     D.t1[i] = t;
     prod = t * D.rh[i];
   }
   const double s = block_tree(prod, lds);
-  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (threadIdx.x == 0) D.part[vb] = s;
 }
 
 // s = r - alpha t1 ; t2 = A s (s at the neighbours formed on the fly) ;
@@ -61,18 +66,17 @@ __global__ void __launch_bounds__(256) k_bicg_1(Dev D) {
 __global__ void __launch_bounds__(256) k_bicg_2(Dev D) {
   __shared__ double lds[RED_BLOCK];
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   const double alpha = D.sc->alpha;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
-    const int cap = D.cap, L = D.alen[i];
-    double acc = 0.0;
-    for (int k = 0; k < L; k++) {
-      const int j = D.acol[k * cap + i];
-      const double sj = D.r[j] - alpha * D.t1[j];
-      acc = acc + D.aval[k * cap + i] * sj;
-    }
+    const double* __restrict__ r = D.r;
+    const double* __restrict__ t1 = D.t1;
+    const double acc = ell_row_sum<ELL_U, double>(
+        D.acol, D.aval, D.cap, i, D.alen[i], [&](int j) { return r[j] - alpha * t1[j]; });
     const double si = D.r[i] - alpha * D.t1[i];
     D.sv[i] = si;
     D.t2[i] = acc;
@@ -82,8 +86,8 @@ __global__ void __launch_bounds__(256) k_bicg_2(Dev D) {
   const double s1 = block_tree(p1, lds);
   const double s2 = block_tree(p2, lds);
   if (threadIdx.x == 0) {
-    D.part[blockIdx.x] = s1;
-    D.part[gridDim.x + blockIdx.x] = s2;
+    D.part[vb] = s1;
+    D.part[D.nblk + vb] = s2;
   }
 }
 
@@ -91,7 +95,9 @@ __global__ void __launch_bounds__(256) k_bicg_2(Dev D) {
 __global__ void __launch_bounds__(256) k_bicg_3(Dev D) {
   __shared__ double lds[RED_BLOCK];
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   const double alpha = D.sc->alpha, omega = D.sc->omega;
   double p1 = 0.0, p2 = 0.0;
@@ -106,29 +112,32 @@ __global__ void __launch_bounds__(256) k_bicg_3(Dev D) {
   const double s1 = block_tree(p1, lds);
   const double s2 = block_tree(p2, lds);
   if (threadIdx.x == 0) {
-    D.part[blockIdx.x] = s1;
-    D.part[gridDim.x + blockIdx.x] = s2;
+    D.part[vb] = s1;
+    D.part[D.nblk + vb] = s2;
   }
 }
 
 // p = r + beta (p - omega t1)
 __global__ void k_bicg_4(Dev D) {
   if (D.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
   const double beta = D.sc->beta, omega = D.sc->omega;
   D.pp[i] = D.r[i] + beta * (D.pp[i] - omega * D.t1[i]);
 }
 
 // generic dot (a . b) into *out through the fixed tree (used for ||b||^2)
-__global__ void __launch_bounds__(256) k_dot(Dev D, const double* a, const double* b,
-                                             double* out) {
+__global__ void __launch_bounds__(256) k_dot(Dev D, const double* a, const double* b) {
   __shared__ double lds[RED_BLOCK];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = row_block();
+  if (vb >= D.nblk) return;
+  const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   const double prod = (i < n) ? a[i] * b[i] : 0.0;
   const double s = block_tree(prod, lds);
-  if (threadIdx.x == 0) D.part[blockIdx.x] = s;
+  if (threadIdx.x == 0) D.part[vb] = s;
 }
 
 // pressure scatter (isph.f90:468-486): the solution vector is indexed by row

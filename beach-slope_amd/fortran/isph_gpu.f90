@@ -221,7 +221,8 @@ module isph_gpu
 
 contains
 
-  ! message of the context's last failure as a Fortran string
+  ! message of the context's last failure as a Fortran string; for a null
+  ! handle, the message of the calling thread's last failed isph_create
   function isph_error_message(ctx) result(msg)
     type(c_ptr), intent(in) :: ctx
     character(len=:), allocatable :: msg

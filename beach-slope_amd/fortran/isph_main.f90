@@ -21,6 +21,7 @@
 ! tid and flags in the same layout, then float64 t, dt.
 program isph_main
   use iso_c_binding
+  use iso_fortran_env, only: output_unit
   use isph_gpu
   implicit none
 
@@ -28,7 +29,7 @@ program isph_main
   type(isph_params) :: prm
   type(isph_scalars) :: sc
   type(isph_solve_stats) :: st
-  type(c_ptr) :: ctx
+  type(c_ptr) :: ctx = c_null_ptr
   character(len=4096) :: inpath, outpath, mode
   integer :: u, j1, iter, ios
   integer(c_int32_t) :: magic_in, version
@@ -72,6 +73,9 @@ program isph_main
   if (ns > 0) read(u) sx(1:ns), sy(1:ns), sm(1:ns), srho(1:ns)
   read(u) time, displaytime
   close(u)
+
+  write(*, '(a, i0, a, i0)') '# case: particles ', n, ', porous solids ', ns
+  flush(output_unit)
 
   ! the state the reference holds when its production loop starts
   call check(isph_create(prm, n, max(ns, 1_c_int64_t), ctx), 'isph_create')
@@ -119,6 +123,7 @@ program isph_main
     if (told < iter * displaytime .and. sc%t >= iter * displaytime) then
       write(*, '(es16.8, es14.6, i8, i12, es16.6, i10)') sc%t, sc%dt, sc%solsteps, &
           st%iterations, st%rel_residual, sc%n
+      flush(output_unit)
       iter = iter + 1
     end if
   end do
@@ -141,7 +146,9 @@ contains
     integer(c_int), intent(in) :: rc
     character(len=*), intent(in) :: what
     if (rc /= ISPH_OK) then
+      ! with a null handle (a failed isph_create) the message is the create's
       write(*, '(3a, i0, 2a)') 'error in ', what, ': code ', rc, ': ', isph_error_message(ctx)
+      flush(output_unit)
       error stop 1
     end if
   end subroutine check

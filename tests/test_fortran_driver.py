@@ -50,6 +50,25 @@ def test_driver_usage_and_bad_file(tmp_path):
     assert r.returncode == 2 and "not an ISPH case file" in r.stdout
 
 
+def test_create_failure_is_reported(pkg, cases, tmp_path):
+    """A rejected isph_create leaves a null handle and its message readable
+    through isph_last_error(NULL): the driver prints it and stops with code 1,
+    the Python host raises it.  (The cell grid is invalid, so no HIP call is
+    made.)"""
+    _need_driver()
+    from beach_slope_amd import caseio
+    c = cases.tiny_box(nx=6, ny=5)
+    bad = dict(c.params, cellx=2)
+    f = tmp_path / "case.bin"
+    caseio.write_case(f, c, time=1e-3, displaytime=1e-3, params=bad)
+    r = subprocess.run([str(DRIVER), str(f), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1, (r.returncode, r.stdout, r.stderr)
+    assert "error in isph_create: code -2" in r.stdout and "cell grid" in r.stdout
+    with pytest.raises(pkg.IsphError, match="cell grid"):
+        pkg.Scenario(c, params=bad)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["phases", "fused"])
 def test_driver_matches_host_steps(pkg, cases, tmp_path, mode):
