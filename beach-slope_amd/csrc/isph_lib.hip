@@ -45,7 +45,6 @@ struct isph_ctx {
   std::vector<void*> allocs;
   double* d_bn2 = nullptr;
   MgDev G{};               // multigrid hierarchy (precision form)
-  double* mg_work = nullptr;
 };
 
 #define HIPCHK(ctx, expr)                                                     \
@@ -227,7 +226,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
     rc |= dalloc(c, &G.bb, off);
     rc |= dalloc(c, &G.rb, off);
     rc |= dalloc(c, &G.dense_inv, MG_DENSE_MAX * MG_DENSE_MAX);
-    rc |= dalloc(c, &c->mg_work, 2 * MG_DENSE_MAX * MG_DENSE_MAX);
+    rc |= dalloc(c, &G.rowst, 9 * N);
     rc |= dalloc(c, &G.d0, N);
     rc |= dalloc(c, &G.x0, N);
     G.w = 0.7f;
@@ -239,8 +238,15 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
     return create_fail(ISPH_EHIP, "isph_create: %s", e.c_str());
   }
   c->G.done = &D.sc->done;
-  if (c->G.nlev >= 1)
-    c->G.e1 = (c->G.nlev >= 2) ? c->G.rb + c->G.lev[1].off : c->G.xb + c->G.lev[1].off;
+  if (params->precision) {
+    // multi-block cell levels while the next level exceeds half a workgroup
+    MgDev& G = c->G;
+    G.nmb = 0;
+    while (G.nmb + 1 < G.nlev && G.lev[G.nmb + 2].n > MG_CO_THREADS / 2) G.nmb++;
+    if (G.nlev >= 2 && G.nmb == 0) G.nmb = 1;
+    // level 1's final correction: rb when level 1 runs multi-block, else xb
+    G.e1 = (G.nmb >= 1) ? G.rb + G.lev[1].off : G.xb + G.lev[1].off;
+  }
   DevScalars s{};
   s.dt = params->dt;
   s.dtsol = params->dt;
@@ -731,12 +737,13 @@ static int mg_apply(isph_ctx* c, const VT* v, float* out) {
                      c->D.d32);
   hipLaunchKernelGGL(k_mg_restrict0, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, c->D, G,
                      (const float*)c->D.d32);
-  if (G.nlev >= 2) {
-    hipLaunchKernelGGL(k_mg_l1_down, dim3(grid_of(G.lev[2].n)), dim3(256), 0, c->stream, G);
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_CO_THREADS), 0, c->stream, G, 2);
-    hipLaunchKernelGGL(k_mg_l1_up, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, G);
-  } else {
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_CO_THREADS), 0, c->stream, G, 1);
+  for (int l = 1; l <= G.nmb; l++)
+    hipLaunchKernelGGL(k_mg_lvl_down, dim3(grid_of(G.lev[l + 1].n)), dim3(256), 0, c->stream, G, l);
+  hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_CO_THREADS), 0, c->stream, G, G.nmb + 1);
+  for (int l = G.nmb; l >= 1; l--) {
+    // the coarse kernel leaves its top level in xb; multi-block levels in rb
+    const float* xc = (l == G.nmb ? G.xb : G.rb) + G.lev[l + 1].off;
+    hipLaunchKernelGGL(k_mg_lvl_up, dim3(grid_of(G.lev[l].n)), dim3(256), 0, c->stream, G, l, xc);
   }
   hipLaunchKernelGGL(k_mg_post0<VT>, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, G, v, out);
   HIPCHK(c, hipGetLastError());
@@ -747,10 +754,11 @@ static int mg_apply(isph_ctx* c, const VT* v, float* out) {
 static int mg_setup(isph_ctx* c) {
   MgDev& G = c->G;
   hipLaunchKernelGGL(k_mg_setup0, dim3(grid_of(c->n_hi)), dim3(256), 0, c->stream, c->D, G);
+  hipLaunchKernelGGL(k_mg_rowst, dim3(grid_of(c->n_hi)), dim3(256), 0, c->stream, c->D, G);
   hipLaunchKernelGGL(k_mg_stencil1, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, c->D, G);
   for (int l = 1; l < G.nlev; l++)
     hipLaunchKernelGGL(k_mg_coarsen, dim3(grid_of(G.lev[l + 1].n)), dim3(256), 0, c->stream, G, l);
-  hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(256), 0, c->stream, G, c->mg_work);
+  hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(256), 0, c->stream, G);
   HIPCHK(c, hipGetLastError());
   return 0;
 }

@@ -15,13 +15,15 @@
 //
 // One V-cycle M(v): x0 = w v/d; r0 = v - A x0; r1 = R r0; e1 = V-cycle on the
 // cell levels; x1 = x0 + c P e1; x = x1 + w (v - A x1)/d, with damped-Jacobi
-// weight w and coarse-correction scale c.  The cell levels (>= 1) run inside
-// one single-workgroup kernel, so a V-cycle is four launches.
+// weight w and coarse-correction scale c.  The fine cell levels run on many
+// workgroups (one down and one up launch per level) until a level has at
+// most half a workgroup of unknowns; the remaining coarse levels and the
+// dense solve run inside one single-workgroup kernel.
 #pragma once
 #include "isph_device.h"
 
 #define MG_MAX_LEVELS 24
-#define MG_DENSE_MAX 128
+#define MG_DENSE_MAX 64
 #define MG_CO_THREADS 1024
 
 struct MgLevel {
@@ -37,11 +39,13 @@ struct MgDev {
   float* bb;                   // per-level right-hand side pool
   float* rb;                   // per-level residual pool
   double* dense_inv;           // MG_DENSE_MAX^2 explicit inverse of the coarsest level
+  double* rowst;               // 9 * cap per-row partial level-1 stencils (setup scratch)
   float* d0;                   // level 0: 1 / (sum of the row's entries in column i)
   float* x0;                   // level 0 pre-smoothed iterate
   float w, cscale;
   const int* done;             // solver's done flag: V-cycles after convergence are no-ops
   float* e1;                   // final level-1 correction read by the level-0 post-smoother
+  int nmb;                     // cell levels 1..nmb run on many workgroups (0: none)
 };
 
 __device__ __forceinline__ int mg_nbr(const MgLevel& L, int I, int k) {
@@ -64,32 +68,50 @@ __global__ void k_mg_setup0(Dev D, MgDev G) {
   G.d0[i] = (d != 0.0 && isfinite(d)) ? (float)(1.0 / d) : 1.0f;
 }
 
-// level 1: Galerkin sums over each cell's particle rows
+// level 1: Galerkin sums over each cell's particle rows, in two passes.
+// Row pass (one thread per particle row): the row's entries summed into the
+// nine neighbour-cell bins of its own cell, in entry order.
+__global__ void k_mg_rowst(Dev D, MgDev G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.sc->n) return;
+  const MgLevel L = G.lev[1];
+  double s[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) s[k] = 0.0;
+  const int cap = D.cap, Lr = D.alen[i];
+  const int I = D.cell[i];
+  const int cjI = I / L.ny, ciI = I - cjI * L.ny;
+  for (int k = 0; k < Lr; k++) {
+    const int j = D.acol[k * cap + i];
+    const int J = D.cell[j];
+    const int cjJ = J / L.ny, ciJ = J - cjJ * L.ny;
+    const int di = ciJ - ciI, dj = cjJ - cjI;
+    if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;  // a lagging particle
+    const double a = D.aval[k * cap + i];
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      if (q == (dj + 1) * 3 + (di + 1)) s[q] = s[q] + a;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; q++) G.rowst[q * cap + i] = s[q];
+}
+
+// Cell pass (one thread per cell): the row sums of the cell's contiguous row
+// range, in row order.  An empty cell gets an identity row.
 __global__ void k_mg_stencil1(Dev D, MgDev G) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   const MgLevel L = G.lev[1];
   if (I >= L.n) return;
-  double s[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) s[k] = 0.0;
-  const int cjI = I / L.ny, ciI = I - cjI * L.ny;
   const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
   const int cap = D.cap;
-  for (int i = i0; i < i1; i++) {
-    const int Lr = D.alen[i];
-    for (int k = 0; k < Lr; k++) {
-      const int j = D.acol[k * cap + i];
-      const int J = D.cell[j];
-      const int cjJ = J / L.ny, ciJ = J - cjJ * L.ny;
-      const int di = ciJ - ciI, dj = cjJ - cjI;
-      if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;  // a lagging particle
-      s[(dj + 1) * 3 + (di + 1)] += D.aval[k * cap + i];
-    }
-  }
   float* st = G.sten + 9 * L.off;
-  if (i1 == i0) s[4] = 1.0;  // empty cell: identity row
 #pragma unroll
-  for (int k = 0; k < 9; k++) st[k * L.n + I] = (float)s[k];
+  for (int q = 0; q < 9; q++) {
+    double s = 0.0;
+    for (int i = i0; i < i1; i++) s = s + G.rowst[q * cap + i];
+    if (q == 4 && i1 == i0) s = 1.0;
+    st[q * L.n + I] = (float)s;
+  }
 }
 
 // level l+1 from level l: sum the 2x2 children's stencils
@@ -124,17 +146,23 @@ __global__ void k_mg_coarsen(MgDev G, int l) {
   for (int k = 0; k < 9; k++) sc[k * C.n + I] = s[k];
 }
 
-// coarsest level: dense matrix and its explicit inverse by Gauss-Jordan with
-// partial pivoting (single workgroup)
-__global__ void __launch_bounds__(256) k_mg_dense(MgDev G, double* work) {
+// Coarsest level: dense matrix and its explicit inverse by Gauss-Jordan with
+// partial pivoting on the augmented matrix [A | I], held in LDS
+// (MG_DENSE_MAX x 2 MG_DENSE_MAX doubles = 64 KB).  Per pivot column: one
+// wavefront finds the pivot (largest magnitude, first row on ties), the rows
+// are swapped, the pivot row scaled, and all other rows updated in parallel
+// from snapshots of the pivot row and column.
+__global__ void __launch_bounds__(256) k_mg_dense(MgDev G) {
+  __shared__ double A[MG_DENSE_MAX * 2 * MG_DENSE_MAX];
+  __shared__ double colc[MG_DENSE_MAX], prow[2 * MG_DENSE_MAX];
+  __shared__ int piv;
   const MgLevel L = G.lev[G.nlev];
-  const int n = L.n, t = threadIdx.x;
-  const float* st = G.sten + 9 * L.off;
-  double* A = work;                    // n x 2n augmented [A | I]
+  const int n = L.n, t = threadIdx.x, T = blockDim.x;
   const int w2 = 2 * n;
-  for (int e = t; e < n * w2; e += blockDim.x) A[e] = 0.0;
+  const float* st = G.sten + 9 * L.off;
+  for (int e = t; e < n * w2; e += T) A[e] = 0.0;
   __syncthreads();
-  for (int I = t; I < n; I += blockDim.x) {
+  for (int I = t; I < n; I += T) {
     for (int k = 0; k < 9; k++) {
       const int J = mg_nbr(L, I, k);
       if (J >= 0) A[I * w2 + J] += (double)st[k * n + I];
@@ -142,39 +170,38 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgDev G, double* work) {
     A[I * w2 + n + I] = 1.0;
   }
   __syncthreads();
-  __shared__ int piv;
-  __shared__ double colc[MG_DENSE_MAX];
   for (int c = 0; c < n; c++) {
-    if (t == 0) {
-      int best = c;
-      double bv = fabs(A[c * w2 + c]);
-      for (int r = c + 1; r < n; r++)
-        if (fabs(A[r * w2 + c]) > bv) { bv = fabs(A[r * w2 + c]); best = r; }
-      piv = best;
+    if (t < 64) {
+      // lane r holds |A(r, c)| for r >= c (n <= 64 rows)
+      double bv = (t >= c && t < n) ? fabs(A[t * w2 + c]) : -1.0;
+      int br = t;
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_down(bv, off, 64);
+        const int orow = __shfl_down(br, off, 64);
+        if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
+      }
+      if (t == 0) piv = br;
     }
     __syncthreads();
-    if (piv != c)
-      for (int e = t; e < w2; e += blockDim.x) {
-        double tmp = A[c * w2 + e];
-        A[c * w2 + e] = A[piv * w2 + e];
-        A[piv * w2 + e] = tmp;
+    const int pr = piv;
+    if (pr != c)
+      for (int e = t; e < w2; e += T) {
+        const double tmp = A[c * w2 + e];
+        A[c * w2 + e] = A[pr * w2 + e];
+        A[pr * w2 + e] = tmp;
       }
     __syncthreads();
     const double pv = A[c * w2 + c];
+    for (int e = t; e < w2; e += T) prow[e] = (pv != 0.0) ? A[c * w2 + e] / pv : 0.0;
+    for (int r = t; r < n; r += T) colc[r] = A[r * w2 + c];
     __syncthreads();
-    for (int e = t; e < w2; e += blockDim.x) A[c * w2 + e] = (pv != 0.0) ? A[c * w2 + e] / pv : 0.0;
-    // snapshot the pivot column before any row is updated
-    for (int r = t; r < n; r += blockDim.x) colc[r] = A[r * w2 + c];
-    __syncthreads();
-    for (int r = 0; r < n; r++) {
-      if (r == c) continue;
-      const double f = colc[r];
-      if (f == 0.0) continue;
-      for (int e = t; e < w2; e += blockDim.x) A[r * w2 + e] = A[r * w2 + e] - f * A[c * w2 + e];
+    for (int e = t; e < n * w2; e += T) {
+      const int r = e / w2, q = e - r * w2;
+      A[e] = (r == c) ? prow[q] : A[e] - colc[r] * prow[q];
     }
     __syncthreads();
   }
-  for (int e = t; e < n * n; e += blockDim.x) {
+  for (int e = t; e < n * n; e += T) {
     const int r = e / n, cc = e - r * n;
     G.dense_inv[e] = A[r * w2 + n + cc];
   }
@@ -239,7 +266,7 @@ __device__ __forceinline__ float mg_apply_row(const MgLevel& L, const float* st,
   return acc;
 }
 
-// levels l0..nlev in one workgroup (l0 = 2 when level 1 runs multi-block)
+// levels l0..nlev in one workgroup (l0 = nmb + 1)
 __global__ void __launch_bounds__(MG_CO_THREADS) k_mg_coarse(MgDev G, int l0) {
   if (*G.done) return;
   const int t = threadIdx.x, T = blockDim.x;
@@ -302,12 +329,14 @@ __global__ void __launch_bounds__(MG_CO_THREADS) k_mg_coarse(MgDev G, int l0) {
   }
 }
 
-// level 1 on many workgroups.  Down: one thread per level-2 aggregate
-// pre-smooths its (up to four) cells, forms their residuals with the
-// neighbours' pre-smoothed values computed on the fly, and restricts.
-__global__ void k_mg_l1_down(MgDev G) {
+// Cell level l (1 <= l < nlev) on many workgroups.  Down: one thread per
+// level-(l+1) aggregate pre-smooths its (up to four) cells, forms their
+// residuals with the neighbours' pre-smoothed values computed on the fly, and
+// restricts.  The arithmetic and its order are those of the single-workgroup
+// kernel below, so a level gives the same values whichever kernel runs it.
+__global__ void k_mg_lvl_down(MgDev G, int l) {
   const int J = blockIdx.x * blockDim.x + threadIdx.x;
-  const MgLevel L = G.lev[1], C = G.lev[2];
+  const MgLevel L = G.lev[l], C = G.lev[l + 1];
   if (*G.done || J >= C.n) return;
   const float w = G.w;
   const float* st = G.sten + 9 * L.off;
@@ -332,18 +361,19 @@ __global__ void k_mg_l1_down(MgDev G) {
   G.bb[C.off + J] = s;
 }
 
-// Up: one thread per cell adds the prolonged level-2 correction and
-// post-smooths, with the neighbours' corrected values formed on the fly; the
-// result goes to e1 (the level-1 residual slot, free by then).
-__global__ void k_mg_l1_up(MgDev G) {
+// Up: one thread per cell of level l adds the prolonged level-(l+1)
+// correction xc and post-smooths, with the neighbours' corrected values
+// formed on the fly.  The result goes to the level's residual slot (free by
+// then), which is where the next finer level, and for level 1 the particle
+// level (e1), read it.
+__global__ void k_mg_lvl_up(MgDev G, int l, const float* __restrict__ xc) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  const MgLevel L = G.lev[1], C = G.lev[2];
+  const MgLevel L = G.lev[l], C = G.lev[l + 1];
   if (*G.done || I >= L.n) return;
   const float w = G.w, c = G.cscale;
   const float* st = G.sten + 9 * L.off;
   const float* b = G.bb + L.off;
   const float* x = G.xb + L.off;
-  const float* xc = G.xb + C.off;
   float acc = 0.0f;
 #pragma unroll
   for (int k = 0; k < 9; k++) {
@@ -355,7 +385,7 @@ __global__ void k_mg_l1_up(MgDev G) {
   }
   const int cj = I / L.ny, ci = I - cj * L.ny;
   const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
-  G.e1[I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+  G.rb[L.off + I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
 }
 
 // ---- preconditioned mixed-precision BiCGStab ----
