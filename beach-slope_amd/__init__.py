@@ -81,7 +81,8 @@ EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_set_scalars", "isph_ppesolve", "isph_ppe_solve", "isph_scalarupdate",
            "isph_step", "isph_dump_cells", "isph_dump_neighbours", "isph_dump_props",
            "isph_dump_ppe", "isph_ppe_set_rhs", "isph_ppe_get_solution",
-           "isph_batch_create", "isph_batch_step", "isph_batch_destroy",
+           "isph_load_system", "isph_batch_create", "isph_batch_step", "isph_batch_stats",
+           "isph_batch_destroy",
            "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read") + tuple("isph_" + p for p in PHASES)
 
 _lib = None
@@ -126,6 +127,7 @@ def lib():
     L.isph_load_system.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.isph_batch_create.argtypes = [i32, C.POINTER(vp), C.POINTER(vp)]
     L.isph_batch_step.argtypes = [vp, i32]
+    L.isph_batch_stats.argtypes = [vp, i32, C.POINTER(SolveStats)]
     L.isph_batch_destroy.argtypes = [vp]
     L.isph_prof_enable.argtypes = [vp, i32]
     L.isph_prof_read.argtypes = [vp, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
@@ -191,6 +193,11 @@ class Scenario:
     @property
     def handle(self):
         return self._ctx
+
+    def last_error(self) -> str:
+        if not self._ctx:
+            return ""
+        return (lib().isph_last_error(self._ctx) or b"").decode()
 
     def set_params(self, **kw):
         self.params.update(kw)
@@ -321,6 +328,48 @@ class Scenario:
         x = np.empty(self.n)
         self._check(lib().isph_ppe_get_solution(self._ctx, _ptr(x)), "isph_ppe_get_solution")
         return x
+
+
+class Batch:
+    """Independent scenarios stepped together (isph_batch_*).
+
+    While the batch exists its scenarios share one stream, and the PPE solves
+    of the precision-form scenarios run as one set of launches over all of
+    them.  Close the batch before closing its scenarios."""
+
+    def __init__(self, scenarios):
+        self.scenarios = list(scenarios)
+        handles = (C.c_void_p * len(self.scenarios))(*[s.handle for s in self.scenarios])
+        self._b = C.c_void_p()
+        rc = lib().isph_batch_create(len(self.scenarios), handles, C.byref(self._b))
+        if rc != 0:
+            raise IsphError(rc, "isph_batch_create", self.scenarios[0].last_error())
+
+    def step(self, nsteps: int = 1, check: bool = True) -> int:
+        rc = lib().isph_batch_step(self._b, nsteps)
+        if check and rc != 0:
+            msgs = "; ".join(f"{k}: {s.last_error()}" for k, s in enumerate(self.scenarios)
+                             if s.last_error())
+            raise IsphError(rc, "isph_batch_step", msgs)
+        return rc
+
+    def stats(self, s: int) -> SolveStats:
+        st = SolveStats()
+        rc = lib().isph_batch_stats(self._b, s, C.byref(st))
+        if rc != 0:
+            raise IsphError(rc, "isph_batch_stats")
+        return st
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().isph_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def footprint_bytes(n_cap: int, n_solid_cap: int, cellx: int, celly: int,

@@ -29,7 +29,7 @@ struct DevParams {
   double h, dl, lam, rho, rel_den, rhomin, g, mu, por, d50, lamk, csh, maxshift;
   double fs_thresh, sig1, tschmidt, xlcutoff, ytcutoff, buf_xmin, buf_xmax;
   double tol, inner_tol;
-  int fixed_dt, max_iter;
+  int fixed_dt, max_iter, max_outer;
   // constants derived on the host with the same expressions as the reference
   double cs;        // incr*h1 = 4h (setup.f90:120-123)
   double k1, k2;    // 10/(28*3.14 h^2), 1/(3*3.14 h^2), real4 products (kernel.f90:61-62)
@@ -45,6 +45,7 @@ struct DevScalars {
   // BiCGStab state
   double rho, alpha, omega, beta, rr, bnorm, stop, rel;
   int iters, done, conv, brk;
+  int fin, restarts;  // precision form: scenario finished; restarts from the FP64 residual
   unsigned int counter[4];
   unsigned long long nnz;  // stored PPE entries of the last assembly
 };

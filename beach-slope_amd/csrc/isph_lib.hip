@@ -18,6 +18,7 @@
 #include "isph_kernels.h"
 #include "isph_solver.h"
 #include "isph_mg.h"
+#include "isph_batch.h"
 
 static_assert(NBR_CAP == ISPH_NBR_CAP, "neighbour capacity mismatch");
 
@@ -45,6 +46,7 @@ struct isph_ctx {
   std::vector<void*> allocs;
   double* d_bn2 = nullptr;
   MgDev G{};               // multigrid hierarchy (precision form)
+  struct BatchHost* self_batch = nullptr;  // this context as a batch of one (precision solve)
 };
 
 #define HIPCHK(ctx, expr)                                                     \
@@ -120,6 +122,7 @@ static void fill_devparams(isph_ctx* c) {
   d.buf_xmin = P.buf_xmin; d.buf_xmax = P.buf_xmax; d.tol = P.tol; d.inner_tol = P.inner_tol;
   d.fixed_dt = P.fixed_dt;
   d.max_iter = P.max_iter;
+  d.max_outer = P.max_outer;
   d.cs = 4.0 * P.h;
   d.k1 = 10.0 / ((double)(28.0f * 3.14f) * (P.h * P.h));
   d.k2 = 1.0 / ((double)(3.0f * 3.14f) * (P.h * P.h));
@@ -138,9 +141,11 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   per += 4 * 3;                                 // bin, rank, scan scratch
   per += 4 + ROW_CAP * (4 + 8) + 2 * 8;         // PPE rows, rhs, solution
   per += 6 * 8;                                 // FP64 solver vectors
-  if (precision) per += ROW_CAP * 4 + 7 * 4;    // FP32 matrix and vectors
+  if (precision) per += ROW_CAP * 4 + 7 * 4 + 2 * 4 + 9 * 8;  // FP32 matrix and vectors,
+                                                             // level-0 multigrid vectors, setup rows
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
   b += 8 * 2 * (n_cap / 256 + 2);               // reduction partials
+  if (precision) b += 4 * 12 * ncell + 8 * MG_DENSE_MAX * MG_DENSE_MAX;  // cell levels
   return b + (1 << 20);
 }
 
@@ -262,9 +267,12 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   return 0;
 }
 
+static void bh_free(BatchHost* b);
+
 extern "C" int isph_destroy(isph_ctx* c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
+  bh_free(c->self_batch);
   for (void* p : c->allocs) (void)hipFree(p);
   for (KTimer& t : c->timers) {
     for (auto& pr : t.pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -728,102 +736,241 @@ extern "C" int isph_ppe_assemble(isph_ctx* c) {
   return check_err_flags(c);
 }
 
-// one V-cycle of the cell multigrid: out = M(v)
-template <typename VT>
-static int mg_apply(isph_ctx* c, const VT* v, float* out) {
-  const int n = c->n_hi;
-  MgDev& G = c->G;
-  hipLaunchKernelGGL(k_mg_pre0<VT>, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, G, v,
-                     c->D.d32);
-  hipLaunchKernelGGL(k_mg_restrict0, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, c->D, G,
-                     (const float*)c->D.d32);
-  for (int l = 1; l <= G.nmb; l++)
-    hipLaunchKernelGGL(k_mg_lvl_down, dim3(grid_of(G.lev[l + 1].n)), dim3(256), 0, c->stream, G, l);
-  hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_CO_THREADS), 0, c->stream, G, G.nmb + 1);
-  for (int l = G.nmb; l >= 1; l--) {
-    // the coarse kernel leaves its top level in xb; multi-block levels in rb
-    const float* xc = (l == G.nmb ? G.xb : G.rb) + G.lev[l + 1].off;
-    hipLaunchKernelGGL(k_mg_lvl_up, dim3(grid_of(G.lev[l].n)), dim3(256), 0, c->stream, G, l, xc);
+// ---------------------------------------------------------------------
+// precision-form solve for a set of scenarios (one context = a batch of one)
+// ---------------------------------------------------------------------
+struct BatchHost {
+  std::vector<isph_ctx*> ctx;
+  int S = 0;
+  Dev* d_devs = nullptr;
+  MgDev* d_mgs = nullptr;
+  int* d_pre = nullptr;
+  DevScalars* d_sc = nullptr;
+  std::vector<int> pre;          // BK_NKIND x (S+1)
+  std::vector<Dev> devs;
+  std::vector<MgDev> mgs;
+  std::vector<DevScalars> sc;
+  int max_nmb = 0, max_nlev = 0;
+};
+
+static void bh_free(BatchHost* b) {
+  if (!b) return;
+  (void)hipFree(b->d_devs);
+  (void)hipFree(b->d_mgs);
+  (void)hipFree(b->d_pre);
+  (void)hipFree(b->d_sc);
+  delete b;
+}
+
+// device tables for the scenarios of ctxs (all precision-form contexts)
+static int bh_make(isph_ctx* const* ctxs, int S, BatchHost** out) {
+  isph_ctx* c = ctxs[0];
+  BatchHost* b = new BatchHost();
+  b->ctx.assign(ctxs, ctxs + S);
+  b->S = S;
+  b->pre.assign((size_t)BK_NKIND * (S + 1), 0);
+  b->devs.resize(S);
+  b->mgs.resize(S);
+  b->sc.resize(S);
+  hipError_t e = hipMalloc((void**)&b->d_devs, sizeof(Dev) * S);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_mgs, sizeof(MgDev) * S);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_pre, sizeof(int) * b->pre.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_sc, sizeof(DevScalars) * S);
+  if (e != hipSuccess) {
+    set_err(c, "batch tables: %s", hipGetErrorString(e));
+    bh_free(b);
+    return ISPH_EHIP;
   }
-  hipLaunchKernelGGL(k_mg_post0<VT>, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, G, v, out);
-  HIPCHK(c, hipGetLastError());
+  // cell-level prefixes are fixed for the life of the contexts
+  for (int s = 0; s < S; s++) {
+    const MgDev& G = ctxs[s]->G;
+    b->max_nmb = std::max(b->max_nmb, G.nmb);
+    b->max_nlev = std::max(b->max_nlev, G.nlev);
+  }
+  for (int s = 0; s < S; s++) {
+    const MgDev& G = ctxs[s]->G;
+    auto put = [&](int kind, int blocks) {
+      b->pre[(size_t)kind * (S + 1) + s + 1] = b->pre[(size_t)kind * (S + 1) + s] + blocks;
+    };
+    put(BK_CELL1, grid_of(G.lev[1].n));
+    for (int l = 1; l < MG_MAX_LEVELS; l++) {
+      put(BK_DOWN + l - 1, (l <= G.nmb) ? grid_of(G.lev[l + 1].n) : 0);
+      put(BK_UP + l - 1, (l <= G.nmb) ? grid_of(G.lev[l].n) : 0);
+      put(BK_COARSEN + l - 1, (l < G.nlev) ? grid_of(G.lev[l + 1].n) : 0);
+    }
+  }
+  *out = b;
   return 0;
 }
 
-// hierarchy of the precision form, rebuilt from the assembled FP64 rows
-static int mg_setup(isph_ctx* c) {
-  MgDev& G = c->G;
-  hipLaunchKernelGGL(k_mg_setup0, dim3(grid_of(c->n_hi)), dim3(256), 0, c->stream, c->D, G);
-  hipLaunchKernelGGL(k_mg_rowst, dim3(grid_of(c->n_hi)), dim3(256), 0, c->stream, c->D, G);
-  hipLaunchKernelGGL(k_mg_stencil1, dim3(grid_of(G.lev[1].n)), dim3(256), 0, c->stream, c->D, G);
-  for (int l = 1; l < G.nlev; l++)
-    hipLaunchKernelGGL(k_mg_coarsen, dim3(grid_of(G.lev[l + 1].n)), dim3(256), 0, c->stream, G, l);
-  hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(256), 0, c->stream, G);
-  HIPCHK(c, hipGetLastError());
+// refresh the scenario tables (state pointers swap at every rebin) and the
+// row prefix (the live count can shrink)
+static int bh_refresh(BatchHost* b) {
+  isph_ctx* c = b->ctx[0];
+  const int S = b->S;
+  for (int s = 0; s < S; s++) {
+    b->devs[s] = b->ctx[s]->D;
+    b->mgs[s] = b->ctx[s]->G;
+    b->pre[(size_t)BK_ROWS * (S + 1) + s + 1] =
+        b->pre[(size_t)BK_ROWS * (S + 1) + s] + b->ctx[s]->D.nblk;
+  }
+  HIPCHK(c, hipMemcpyAsync(b->d_devs, b->devs.data(), sizeof(Dev) * S, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(b->d_mgs, b->mgs.data(), sizeof(MgDev) * S, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(b->d_pre, b->pre.data(), sizeof(int) * b->pre.size(),
+                           hipMemcpyHostToDevice, c->stream));
   return 0;
 }
 
-#define PFINALIZE(op, nrows, init)                                                   \
-  do {                                                                               \
-    hipLaunchKernelGGL(k_pfinalize, dim3(1), dim3(256), 0, c->stream, c->D, (op),    \
-                       grid_of(nrows), (init));                                      \
-    HIPCHK(c, hipGetLastError());                                                    \
+static int bh_gather(BatchHost* b) {
+  isph_ctx* c = b->ctx[0];
+  BatchDev B{b->S, b->d_devs, b->d_mgs, b->d_pre};
+  hipLaunchKernelGGL(kb_gather_scalars, dim3((b->S + 63) / 64), dim3(64), 0, c->stream, B, b->d_sc);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(b->sc.data(), b->d_sc, sizeof(DevScalars) * b->S,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+#define BLAUNCH(kern, grid, block, ...)                                               \
+  do {                                                                                \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, __VA_ARGS__);     \
+    hipError_t e_ = hipGetLastError();                                                \
+    if (e_ != hipSuccess) {                                                           \
+      set_err(c, "launch of %s failed: %s", #kern, hipGetErrorString(e_));           \
+      return ISPH_EHIP;                                                               \
+    }                                                                                 \
   } while (0)
 
-// precision form: cell-multigrid right-preconditioned BiCGStab with FP32
-// matrix, V-cycles and search directions; FP64 solution, residual and dots.
-// A recursive-residual convergence claim is checked against b - A64 x; when
-// it fails the iteration restarts from the FP64 residual.
-static int pbicg_solve(isph_ctx* c, isph_solve_stats* st) {
-  const int n = c->n_hi;
-  int rc = mg_setup(c);
+// A timed batched row launch (events attached to the dispatch, as TIMED_LAUNCH)
+#define TIMED_BLAUNCH(tidx, kern, grid, ...)                                          \
+  do {                                                                                \
+    if (c->prof) {                                                                    \
+      std::pair<hipEvent_t, hipEvent_t> ev_ = timer_pair(c->timers[tidx]);            \
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, ev_.first,     \
+                            ev_.second, 0, __VA_ARGS__);                              \
+      hipError_t e_ = hipGetLastError();                                              \
+      if (e_ != hipSuccess) {                                                         \
+        set_err(c, "launch of %s failed: %s", #kern, hipGetErrorString(e_));         \
+        return ISPH_EHIP;                                                             \
+      }                                                                               \
+      c->timers[tidx].pending.push_back(ev_);                                         \
+    } else {                                                                          \
+      BLAUNCH(kern, grid, 256, __VA_ARGS__);                                          \
+    }                                                                                 \
+  } while (0)
+
+// one V-cycle for every scenario: WHICH 0 gives r32 = M(pp32), 1 sv32 = M(sv)
+template <int WHICH>
+static int bh_vcycle(BatchHost* b, const BatchDev& B, int grow) {
+  isph_ctx* c = b->ctx[0];
+  const int S = b->S;
+  auto tot = [&](int kind) { return b->pre[(size_t)kind * (S + 1) + S]; };
+  BLAUNCH(kb_mg_pre0<WHICH>, grow, 256, B);
+  BLAUNCH(kb_mg_restrict0, tot(BK_CELL1), 256, B);
+  for (int l = 1; l <= b->max_nmb; l++)
+    if (tot(BK_DOWN + l - 1) > 0) BLAUNCH(kb_mg_down, tot(BK_DOWN + l - 1), 256, B, l);
+  BLAUNCH(kb_mg_coarse, S, MG_CO_THREADS, B);
+  for (int l = b->max_nmb; l >= 1; l--)
+    if (tot(BK_UP + l - 1) > 0) BLAUNCH(kb_mg_up, tot(BK_UP + l - 1), 256, B, l);
+  BLAUNCH(kb_mg_post0<WHICH>, grow, 256, B);
+  return 0;
+}
+
+// Precision-form solve of every scenario of b: cell-multigrid right-
+// preconditioned BiCGStab with FP32 matrix, V-cycles and search directions;
+// FP64 solution, residual and dots.  A recursive-residual convergence claim
+// is checked against b - A64 x after the chunk in which it was made; when the
+// check fails the iteration continues from the FP64 residual.  The host only
+// polls whether every scenario has finished; stats[s] gets scenario s's.
+static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
+  isph_ctx* c = b->ctx[0];
+  const int S = b->S;
+  int rc = bh_refresh(b);
   if (rc) return rc;
-  float* ph = c->D.r32;
-  float* sh = c->D.sv32;
-  hipLaunchKernelGGL(k_pb_resid, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, 1);
-  PFINALIZE(PFIN_RESID, n, 1);
-  DevScalars s;
-  rc = read_scalars(c, &s);
+  const BatchDev B{S, b->d_devs, b->d_mgs, b->d_pre};
+  auto tot = [&](int kind) { return b->pre[(size_t)kind * (S + 1) + S]; };
+  const int grow = 8 * ((tot(BK_ROWS) + 7) / 8);
+  // multigrid hierarchy from the assembled rows
+  BLAUNCH(kb_mg_setup0, grow, 256, B);
+  BLAUNCH(kb_mg_rowst, grow, 256, B);
+  BLAUNCH(kb_mg_stencil1, tot(BK_CELL1), 256, B);
+  for (int l = 1; l < b->max_nlev; l++)
+    if (tot(BK_COARSEN + l - 1) > 0) BLAUNCH(kb_mg_coarsen, tot(BK_COARSEN + l - 1), 256, B, l);
+  BLAUNCH(kb_mg_dense, S, 256, B);
+  // ||b||, then the FP64 start residual
+  BLAUNCH(kb_pb_bnorm, grow, 256, B);
+  BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BNORM, 0);
+  BLAUNCH(kb_pb_resid, grow, 256, B, 1);
+  BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 1);
+  rc = bh_gather(b);
   if (rc) return rc;
-  int restarts = 0, chunk = 4;
-  int it0 = s.iters;  // iterations before this restart cycle
-  while (!s.conv) {
-    for (int k = 0; k < chunk; k++) {
-      rc = mg_apply<float>(c, c->D.pp32, ph);
-      if (rc) return rc;
-      TIMED_LAUNCH(2, k_pb_1, n, c->D, c->G, (const float*)ph);
-      PFINALIZE(PFIN_ALPHA, n, 0);
-      RLAUNCH(k_pb_s, n, c->D);
-      rc = mg_apply<double>(c, c->D.sv, sh);
-      if (rc) return rc;
-      TIMED_LAUNCH(3, k_pb_2, n, c->D, c->G, (const float*)sh);
-      PFINALIZE(PFIN_OMEGA, n, 0);
-      RLAUNCH(k_pb_3, n, c->D, (const float*)ph, (const float*)sh);
-      PFINALIZE(PFIN_BETA, n, 0);
-      RLAUNCH(k_pb_4, n, c->D);
-    }
-    rc = read_scalars(c, &s);
-    if (rc) return rc;
-    if (!s.done) {
-      if (chunk < 16) chunk *= 2;
-      continue;
-    }
-    if (c->prof) timers_collect(c, s.iters - it0);
-    // the FP64 residual decides; restart from it unless it is met
-    hipLaunchKernelGGL(k_pb_resid, dim3(row_grid(n)), dim3(256), 0, c->stream, c->D, 2);
-    PFINALIZE(PFIN_RESID, n, 2);
-    rc = read_scalars(c, &s);
-    if (rc) return rc;
-    if (s.conv) break;
-    if (s.iters >= c->P.max_iter || ++restarts > c->P.max_outer) break;
-    chunk = 4;
-    it0 = s.iters;
+  std::vector<int> it_prev(S), rs_prev(S);
+  auto all_fin = [&]() {
+    for (int s = 0; s < S; s++)
+      if (!b->sc[s].fin) return false;
+    return true;
+  };
+  int max_iter = 0, max_outer = 0;
+  for (int s = 0; s < S; s++) {
+    max_iter = std::max(max_iter, b->ctx[s]->P.max_iter);
+    max_outer = std::max(max_outer, b->ctx[s]->P.max_outer);
+    it_prev[s] = b->sc[s].iters;
+    rs_prev[s] = b->sc[s].restarts;
   }
-  st->iterations = s.iters;
-  st->outer = restarts;
-  st->converged = s.conv;
-  st->breakdown = s.brk;
-  st->rel_residual = s.rel;
+  // every chunk either advances an unfinished scenario by at least one
+  // iteration or ends it at the check, so this bound is never the exit
+  const int64_t chunk_cap = 4 + (int64_t)(max_iter + 1) * (max_outer + 2);
+  int chunk = 4;
+  for (int64_t nchunks = 0; !all_fin(); nchunks++) {
+    if (nchunks > chunk_cap) {
+      set_err(c, "precision solve: scenarios did not finish");
+      return ISPH_ENOCONV;
+    }
+    for (int k = 0; k < chunk; k++) {
+      rc = bh_vcycle<0>(b, B, grow);
+      if (rc) return rc;
+      TIMED_BLAUNCH(2, kb_pb_1, grow, B);
+      BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_ALPHA, 0);
+      BLAUNCH(kb_pb_s, grow, 256, B);
+      rc = bh_vcycle<1>(b, B, grow);
+      if (rc) return rc;
+      TIMED_BLAUNCH(3, kb_pb_2, grow, B);
+      BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_OMEGA, 0);
+      BLAUNCH(kb_pb_3, grow, 256, B);
+      BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BETA, 0);
+      BLAUNCH(kb_pb_4, grow, 256, B);
+    }
+    // FP64 check of every scenario whose iteration stopped in this chunk
+    BLAUNCH(kb_pb_resid, grow, 256, B, 2);
+    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 2);
+    rc = bh_gather(b);
+    if (rc) return rc;
+    // the timed launches that did work: the chunk's leading iterations
+    int worked = 0, restarted = 0;
+    for (int s = 0; s < S; s++) {
+      worked = std::max(worked, b->sc[s].iters - it_prev[s]);
+      restarted |= (b->sc[s].restarts != rs_prev[s]);
+      it_prev[s] = b->sc[s].iters;
+      rs_prev[s] = b->sc[s].restarts;
+    }
+    if (c->prof) timers_collect(c, worked);
+    chunk = restarted ? 4 : std::min(16, 2 * chunk);
+  }
+  BLAUNCH(kb_copy_sol_to_p, grow, 256, B);
+  for (int s = 0; s < S; s++) {
+    const DevScalars& d = b->sc[s];
+    isph_solve_stats& st = stats[s];
+    st.iterations = d.iters;
+    st.outer = d.restarts;
+    st.converged = d.conv;
+    st.breakdown = d.brk;
+    st.rel_residual = d.rel;
+    st.bnorm = d.bnorm;
+    st.nnz = (double)d.nnz;
+  }
   return 0;
 }
 
@@ -865,53 +1012,57 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
   }
   const int n = c->n_hi;
   isph_solve_stats st{};
-  {
-    long long nnz = 0;
-    HIPCHK(c, hipMemcpyAsync(&nnz, &c->D.sc->nnz, sizeof nnz, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    st.nnz = (double)nnz;
-  }
-  // ||b||
-  RLAUNCH(k_dot, n, c->D, c->D.b, c->D.b);
-  FINALIZE_OUT(FIN_DOT, n, c->d_bn2);
-  double bn2 = 0;
-  HIPCHK(c, hipMemcpyAsync(&bn2, c->d_bn2, 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  const double bnorm = std::sqrt(bn2);
-  st.bnorm = bnorm;
-  DevScalars s;
-  int rc = read_scalars(c, &s);
-  if (rc) return rc;
-  s.bnorm = bnorm;
-  HIPCHK(c, hipMemcpy(&c->D.sc->bnorm, &bnorm, 8, hipMemcpyHostToDevice));
-  if (bnorm == 0.0 || !std::isfinite(bnorm)) {
-    if (bnorm == 0.0) {
-      HIPCHK(c, hipMemsetAsync(c->D.sol, 0, 8 * (size_t)c->cap, c->stream));
-      st.converged = 1;
-    }
-  } else if (c->P.precision == ISPH_PPE_FP64) {
-    RLAUNCH(k_bicg_init, n, c->D);
-    FINALIZE(FIN_INIT, n);
-    rc = read_scalars(c, &s);
-    if (rc) return rc;
-    int it = 0, brk = 0, conv = s.conv;
-    double rel = s.rel;
-    if (!s.done) {
-      rc = bicg_loop(c, &it, &brk, &conv, &rel);
+  if (c->P.precision != ISPH_PPE_FP64) {
+    if (!c->self_batch) {
+      int rc = bh_make(&c, 1, &c->self_batch);
       if (rc) return rc;
     }
-    st.iterations = it;
-    st.breakdown = brk;
-    st.converged = conv;
-    st.rel_residual = rel;
-  } else {
-    rc = pbicg_solve(c, &st);
+    int rc = bh_solve(c->self_batch, &st);
     if (rc) return rc;
+  } else {
+    {
+      long long nnz = 0;
+      HIPCHK(c, hipMemcpyAsync(&nnz, &c->D.sc->nnz, sizeof nnz, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      st.nnz = (double)nnz;
+    }
+    // ||b||
+    RLAUNCH(k_dot, n, c->D, c->D.b, c->D.b);
+    FINALIZE_OUT(FIN_DOT, n, c->d_bn2);
+    double bn2 = 0;
+    HIPCHK(c, hipMemcpyAsync(&bn2, c->d_bn2, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double bnorm = std::sqrt(bn2);
+    st.bnorm = bnorm;
+    HIPCHK(c, hipMemcpy(&c->D.sc->bnorm, &bnorm, 8, hipMemcpyHostToDevice));
+    if (bnorm == 0.0 || !std::isfinite(bnorm)) {
+      if (bnorm == 0.0) {
+        HIPCHK(c, hipMemsetAsync(c->D.sol, 0, 8 * (size_t)c->cap, c->stream));
+        st.converged = 1;
+      }
+      st.rel_residual = (bnorm == 0.0) ? 0.0 : bnorm;
+    } else {
+      RLAUNCH(k_bicg_init, n, c->D);
+      FINALIZE(FIN_INIT, n);
+      DevScalars s;
+      int rc = read_scalars(c, &s);
+      if (rc) return rc;
+      int it = 0, brk = 0, conv = s.conv;
+      double rel = s.rel;
+      if (!s.done) {
+        rc = bicg_loop(c, &it, &brk, &conv, &rel);
+        if (rc) return rc;
+      }
+      st.iterations = it;
+      st.breakdown = brk;
+      st.converged = conv;
+      st.rel_residual = rel;
+    }
+    LAUNCH(k_copy_sol_to_p, n, c->D);
   }
-  LAUNCH(k_copy_sol_to_p, n, c->D);
   c->last = st;
   if (stats) *stats = st;
-  if (!std::isfinite(st.rel_residual) || !std::isfinite(bnorm)) {
+  if (!std::isfinite(st.rel_residual) || !std::isfinite(st.bnorm)) {
     set_err(c, "non-finite PPE residual");
     return ISPH_ENONFINITE;
   }
@@ -1011,8 +1162,9 @@ __global__ void k_advance_t(Dev D) {
   if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->t = D.sc->t + D.sc->dt;
 }
 
-// one production-loop iteration, main.f90:199-261
-static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
+// one production-loop iteration, main.f90:199-261, in three parts: up to the
+// PPE assembly, the solve, and the rest
+static int step_pre(isph_ctx* c) {
   DevScalars s;
   int rc = read_scalars(c, &s);  // solsteps from the previous timestep
   if (rc) return rc;
@@ -1037,14 +1189,13 @@ static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
   LAUNCH(k_int_vel, c->n_hi, c->D);
   rc = assemble_nocheck(c);
   if (rc) return rc;
-  rc = check_err_flags(c);
-  if (rc) return rc;
-  rc = ppe_solve_impl(c, stats);
-  if (rc == ISPH_ENONFINITE || rc == ISPH_EHIP || rc == ISPH_ESTATE) return rc;
-  *solve_rc = rc;
+  return check_err_flags(c);
+}
+
+static int step_post(isph_ctx* c) {
   LAUNCH(k_comp_vel, c->n_hi, c->D);
   LAUNCH(k_comp_pos, c->n_hi, c->D);
-  rc = rebin(c, 1);
+  int rc = rebin(c, 1);
   if (rc) return rc;
   rc = neighbour_nocheck(c);
   if (rc) return rc;
@@ -1057,6 +1208,20 @@ static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
   if (rc) return rc;
   LAUNCH(k_eddyvis, c->n_hi, c->D);
   return check_err_flags(c);
+}
+
+// a solve result that ends the step: errors other than non-convergence
+static bool solve_fatal(int rc) {
+  return rc == ISPH_ENONFINITE || rc == ISPH_EHIP || rc == ISPH_ESTATE || rc == ISPH_EARG;
+}
+
+static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
+  int rc = step_pre(c);
+  if (rc) return rc;
+  rc = ppe_solve_impl(c, stats);
+  if (solve_fatal(rc)) return rc;
+  *solve_rc = rc;
+  return step_post(c);
 }
 
 extern "C" int isph_step(isph_ctx* c, int32_t nsteps, isph_solve_stats* stats) {
@@ -1207,27 +1372,94 @@ extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, cons
 // ---------------------------------------------------------------------
 // batched form: independent scenarios advanced together
 // ---------------------------------------------------------------------
+// Batched form: independent scenarios stepped together.  Every context of a
+// batch is switched to one shared stream; the phases before and after the
+// PPE solve are launched per scenario, and the precision-form solve of all
+// scenarios runs as one set of launches (bh_solve).  Contexts in the FP64
+// form are solved one after the other.
 struct isph_batch {
   std::vector<isph_ctx*> ctx;
+  std::vector<hipStream_t> own;  // each context's stream, restored on destroy
+  hipStream_t stream = nullptr;
+  BatchHost* pb = nullptr;       // the precision-form contexts as one solve batch
+  std::vector<isph_ctx*> pctx;
+  std::vector<isph_solve_stats> last;
 };
 
 extern "C" int isph_batch_create(int32_t n_scen, isph_ctx* const* ctxs, isph_batch** out) {
+  if (out) *out = nullptr;
   if (n_scen <= 0 || !ctxs || !out) return ISPH_EARG;
+  for (int s = 0; s < n_scen; s++) {
+    if (!ctxs[s]) return ISPH_EARG;
+    for (int q = 0; q < s; q++)
+      if (ctxs[q] == ctxs[s]) return ISPH_EARG;
+    if (ctxs[s]->device != ctxs[0]->device) {
+      set_err(ctxs[s], "batched contexts must share one device");
+      return ISPH_EARG;
+    }
+  }
   isph_batch* b = new isph_batch();
   b->ctx.assign(ctxs, ctxs + n_scen);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return ISPH_EHIP;
+  }
+  for (isph_ctx* c : b->ctx) {
+    (void)hipStreamSynchronize(c->stream);
+    b->own.push_back(c->stream);
+    c->stream = b->stream;
+    if (c->P.precision != ISPH_PPE_FP64) b->pctx.push_back(c);
+  }
+  if (!b->pctx.empty()) {
+    int rc = bh_make(b->pctx.data(), (int)b->pctx.size(), &b->pb);
+    if (rc) {
+      isph_batch_destroy(b);
+      return rc;
+    }
+  }
+  b->last.resize(n_scen);
   *out = b;
   return 0;
 }
 
 extern "C" int isph_batch_step(isph_batch* b, int32_t nsteps) {
-  if (!b) return ISPH_EARG;
+  if (!b || nsteps < 0) return ISPH_EARG;
   int worst = 0;
+  std::vector<isph_solve_stats> pst(b->pctx.size());
   for (int k = 0; k < nsteps; k++) {
     for (isph_ctx* c : b->ctx) {
-      int srcode = 0;
-      int rc = step_once(c, nullptr, &srcode);
+      int rc = step_pre(c);
       if (rc) return rc;
-      if (srcode) worst = srcode;
+    }
+    if (b->pb) {
+      int rc = bh_solve(b->pb, pst.data());
+      if (rc) return rc;
+    }
+    size_t q = 0;
+    for (size_t s = 0; s < b->ctx.size(); s++) {
+      isph_ctx* c = b->ctx[s];
+      isph_solve_stats& st = b->last[s];
+      int rc = 0;
+      if (c->P.precision != ISPH_PPE_FP64) {
+        st = pst[q++];
+        c->last = st;
+        if (!std::isfinite(st.rel_residual) || !std::isfinite(st.bnorm)) {
+          set_err(c, "non-finite PPE residual");
+          rc = ISPH_ENONFINITE;
+        } else if (!st.converged) {
+          set_err(c, "PPE solve did not converge: %d iterations, rel residual %g",
+                  st.iterations, st.rel_residual);
+          rc = ISPH_ENOCONV;
+        }
+      } else {
+        rc = ppe_solve_impl(c, &st);
+      }
+      if (solve_fatal(rc)) return rc;
+      if (rc) worst = rc;
+    }
+    for (isph_ctx* c : b->ctx) {
+      int rc = step_post(c);
+      if (rc) return rc;
     }
   }
   for (isph_ctx* c : b->ctx) {
@@ -1237,7 +1469,18 @@ extern "C" int isph_batch_step(isph_batch* b, int32_t nsteps) {
   return worst;
 }
 
+extern "C" int isph_batch_stats(isph_batch* b, int32_t s, isph_solve_stats* out) {
+  if (!b || !out || s < 0 || s >= (int32_t)b->ctx.size()) return ISPH_EARG;
+  *out = b->last[s];
+  return 0;
+}
+
 extern "C" int isph_batch_destroy(isph_batch* b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (size_t s = 0; s < b->own.size(); s++) b->ctx[s]->stream = b->own[s];
+  bh_free(b->pb);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
 }

@@ -1,5 +1,7 @@
-// isph_mg.h -- cell-aggregation multigrid preconditioner for the precision
-// form of the PPE solve.
+// isph_mg.h -- cell-aggregation multigrid preconditioner and the
+// preconditioned mixed-precision BiCGStab of the precision form: data layout
+// and per-scenario device bodies.  The kernels that run them, one launch per
+// phase for any number of scenarios, are in isph_batch.h.
 //
 // Level 0 is the particle matrix itself (FP32 copy of the canonical rows).
 // Level 1 aggregates every cell's particles into one unknown: its operator is
@@ -16,9 +18,9 @@
 // One V-cycle M(v): x0 = w v/d; r0 = v - A x0; r1 = R r0; e1 = V-cycle on the
 // cell levels; x1 = x0 + c P e1; x = x1 + w (v - A x1)/d, with damped-Jacobi
 // weight w and coarse-correction scale c.  The fine cell levels run on many
-// workgroups (one down and one up launch per level) until a level has at
-// most half a workgroup of unknowns; the remaining coarse levels and the
-// dense solve run inside one single-workgroup kernel.
+// workgroups (one down and one up pass per level) until a level has at most
+// half a workgroup of unknowns; the remaining coarse levels and the dense
+// solve run inside one workgroup per scenario.
 #pragma once
 #include "isph_device.h"
 
@@ -33,6 +35,7 @@ struct MgLevel {
 
 struct MgDev {
   int nlev;                    // cell levels 1..nlev (index 0 unused)
+  int nmb;                     // cell levels 1..nmb run on many workgroups (0: none)
   MgLevel lev[MG_MAX_LEVELS];
   float* sten;                 // 9 * (sum of n) stencil pool, [k*n_l + I] at off*9
   float* xb;                   // per-level solution pool
@@ -45,7 +48,6 @@ struct MgDev {
   float w, cscale;
   const int* done;             // solver's done flag: V-cycles after convergence are no-ops
   float* e1;                   // final level-1 correction read by the level-0 post-smoother
-  int nmb;                     // cell levels 1..nmb run on many workgroups (0: none)
 };
 
 __device__ __forceinline__ int mg_nbr(const MgLevel& L, int I, int k) {
@@ -56,10 +58,12 @@ __device__ __forceinline__ int mg_nbr(const MgLevel& L, int I, int k) {
   return b * L.ny + a;
 }
 
-// ---- setup ----
-// level 0: inverse row diagonal (diagonal plus the slot-1 value of ghost rows)
-__global__ void k_mg_setup0(Dev D, MgDev G) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// ---------------------------------------------------------------------
+// setup (once per solve)
+// ---------------------------------------------------------------------
+// level 0, row i: inverse row diagonal (diagonal plus the slot-1 value of
+// ghost rows)
+__device__ __forceinline__ void mg_setup0_row(const Dev& D, const MgDev& G, int i) {
   if (i >= D.sc->n) return;
   const int cap = D.cap, L = D.alen[i];
   double d = 0.0;
@@ -68,11 +72,9 @@ __global__ void k_mg_setup0(Dev D, MgDev G) {
   G.d0[i] = (d != 0.0 && isfinite(d)) ? (float)(1.0 / d) : 1.0f;
 }
 
-// level 1: Galerkin sums over each cell's particle rows, in two passes.
-// Row pass (one thread per particle row): the row's entries summed into the
-// nine neighbour-cell bins of its own cell, in entry order.
-__global__ void k_mg_rowst(Dev D, MgDev G) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// level 1, row pass: row i's entries summed into the nine neighbour-cell bins
+// of its own cell, in entry order
+__device__ __forceinline__ void mg_rowst_row(const Dev& D, const MgDev& G, int i) {
   if (i >= D.sc->n) return;
   const MgLevel L = G.lev[1];
   double s[9];
@@ -96,10 +98,9 @@ __global__ void k_mg_rowst(Dev D, MgDev G) {
   for (int q = 0; q < 9; q++) G.rowst[q * cap + i] = s[q];
 }
 
-// Cell pass (one thread per cell): the row sums of the cell's contiguous row
-// range, in row order.  An empty cell gets an identity row.
-__global__ void k_mg_stencil1(Dev D, MgDev G) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+// level 1, cell pass: the row sums of cell I's contiguous row range, in row
+// order; an empty cell gets an identity row
+__device__ __forceinline__ void mg_stencil1_cell(const Dev& D, const MgDev& G, int I) {
   const MgLevel L = G.lev[1];
   if (I >= L.n) return;
   const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
@@ -114,9 +115,9 @@ __global__ void k_mg_stencil1(Dev D, MgDev G) {
   }
 }
 
-// level l+1 from level l: sum the 2x2 children's stencils
-__global__ void k_mg_coarsen(MgDev G, int l) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+// level l+1 cell I from level l: sum of the 2x2 children's stencils
+__device__ __forceinline__ void mg_coarsen_cell(const MgDev& G, int l, int I) {
+  if (l >= G.nlev) return;
   const MgLevel F = G.lev[l], C = G.lev[l + 1];
   if (I >= C.n) return;
   const int CJ = I / C.ny, CI = I - CJ * C.ny;
@@ -124,19 +125,21 @@ __global__ void k_mg_coarsen(MgDev G, int l) {
   float s[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) s[k] = 0.0f;
-  int nchild = 0;
   for (int b = 0; b < 2; b++) {
     for (int a = 0; a < 2; a++) {
       const int ci = 2 * CI + a, cj = 2 * CJ + b;
       if (ci >= F.ny || cj >= F.nx) continue;
-      nchild++;
       const int f = cj * F.ny + ci;
+#pragma unroll
       for (int k = 0; k < 9; k++) {
         const int di = (k % 3) - 1, dj = (k / 3) - 1;
         const int ni = ci + di, nj = cj + dj;
         if (ni < 0 || ni >= F.ny || nj < 0 || nj >= F.nx) continue;
         const int pi = (ni >> 1) - CI, pj = (nj >> 1) - CJ;
-        s[(pj + 1) * 3 + (pi + 1)] += sf[k * F.n + f];
+        const float v = sf[k * F.n + f];
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+          if (q == (pj + 1) * 3 + (pi + 1)) s[q] += v;
       }
     }
   }
@@ -146,13 +149,13 @@ __global__ void k_mg_coarsen(MgDev G, int l) {
   for (int k = 0; k < 9; k++) sc[k * C.n + I] = s[k];
 }
 
-// Coarsest level: dense matrix and its explicit inverse by Gauss-Jordan with
-// partial pivoting on the augmented matrix [A | I], held in LDS
-// (MG_DENSE_MAX x 2 MG_DENSE_MAX doubles = 64 KB).  Per pivot column: one
+// Coarsest level (one workgroup): dense matrix and its explicit inverse by
+// Gauss-Jordan with partial pivoting on the augmented matrix [A | I], held in
+// LDS (MG_DENSE_MAX x 2 MG_DENSE_MAX doubles = 64 KB).  Per pivot column: one
 // wavefront finds the pivot (largest magnitude, first row on ties), the rows
 // are swapped, the pivot row scaled, and all other rows updated in parallel
 // from snapshots of the pivot row and column.
-__global__ void __launch_bounds__(256) k_mg_dense(MgDev G) {
+__device__ __forceinline__ void mg_dense_block(const MgDev& G) {
   __shared__ double A[MG_DENSE_MAX * 2 * MG_DENSE_MAX];
   __shared__ double colc[MG_DENSE_MAX], prow[2 * MG_DENSE_MAX];
   __shared__ int piv;
@@ -207,14 +210,15 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgDev G) {
   }
 }
 
-// ---- V-cycle, level 0 ----
-// x0 = w v d ; r0 = v - A x0 (x0 of the neighbours formed on the fly)
+// ---------------------------------------------------------------------
+// V-cycle
+// ---------------------------------------------------------------------
+// level 0, row i: x0 = w v d ; r0 = v - A x0 (x0 of the neighbours formed on
+// the fly)
 template <typename VT>
-__global__ void __launch_bounds__(256) k_mg_pre0(Dev D, MgDev G, const VT* __restrict__ v,
-                                                 float* __restrict__ r0) {
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
-  const int i = vb * RED_BLOCK + threadIdx.x;
+__device__ __forceinline__ void mg_pre0_row(const Dev& D, const MgDev& G,
+                                            const VT* __restrict__ v, float* __restrict__ r0,
+                                            int i) {
   if (*G.done || i >= D.sc->n) return;
   const float w = G.w;
   const float* __restrict__ d0 = G.d0;
@@ -225,9 +229,9 @@ __global__ void __launch_bounds__(256) k_mg_pre0(Dev D, MgDev G, const VT* __res
   r0[i] = vi - acc;
 }
 
-// r1 = R r0: sum of each cell's contiguous range
-__global__ void k_mg_restrict0(Dev D, MgDev G, const float* __restrict__ r0) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+// level 1, cell I: r1 = R r0, the sum of the cell's contiguous row range
+__device__ __forceinline__ void mg_restrict0_cell(const Dev& D, const MgDev& G,
+                                                  const float* __restrict__ r0, int I) {
   const MgLevel L = G.lev[1];
   if (*G.done || I >= L.n) return;
   float s = 0.0f;
@@ -236,25 +240,67 @@ __global__ void k_mg_restrict0(Dev D, MgDev G, const float* __restrict__ r0) {
   G.bb[L.off + I] = s;
 }
 
-// x = x1 + w (v - A x1) d with x1 = x0 + c P e1 (x1 of the neighbours on the fly)
-template <typename VT>
-__global__ void __launch_bounds__(256) k_mg_post0(Dev D, MgDev G, const VT* __restrict__ v,
-                                                  float* __restrict__ out) {
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
-  const int i = vb * RED_BLOCK + threadIdx.x;
-  if (*G.done || i >= D.sc->n) return;
-  const float c = G.cscale;
-  const float* __restrict__ e1 = G.e1;
-  const float* __restrict__ x0 = G.x0;
-  const int32_t* __restrict__ cell = D.cell;
-  const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
-                                              [&](int j) { return x0[j] + c * e1[cell[j]]; });
-  const float x1 = G.x0[i] + c * e1[D.cell[i]];
-  out[i] = x1 + G.w * ((float)v[i] - acc) * G.d0[i];
+// Cell level l (1 <= l <= nmb), down pass, aggregate J of level l+1: the
+// thread pre-smooths the aggregate's (up to four) cells, forms their
+// residuals with the neighbours' pre-smoothed values computed on the fly, and
+// restricts.  The arithmetic and its order are those of the single-workgroup
+// pass below, so a level gives the same values whichever pass runs it.
+__device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int J) {
+  if (*G.done || l > G.nmb) return;
+  const MgLevel L = G.lev[l], C = G.lev[l + 1];
+  if (J >= C.n) return;
+  const float w = G.w;
+  const float* st = G.sten + 9 * L.off;
+  const float* b = G.bb + L.off;
+  float* x = G.xb + L.off;
+  const int CJ = J / C.ny, CI = J - CJ * C.ny;
+  float s = 0.0f;
+  for (int bq = 0; bq < 2; bq++)
+    for (int a = 0; a < 2; a++) {
+      const int ci = 2 * CI + a, cj = 2 * CJ + bq;
+      if (ci >= L.ny || cj >= L.nx) continue;
+      const int I = cj * L.ny + ci;
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int N = mg_nbr(L, I, k);
+        if (N >= 0) acc = acc + st[k * L.n + I] * (w * b[N] / st[4 * L.n + N]);
+      }
+      x[I] = w * b[I] / st[4 * L.n + I];
+      s = s + (b[I] - acc);
+    }
+  G.bb[C.off + J] = s;
 }
 
-// ---- V-cycle on the cell levels, one workgroup ----
+// Cell level l (1 <= l <= nmb), up pass, cell I: adds the prolonged
+// level-(l+1) correction and post-smooths, with the neighbours' corrected
+// values formed on the fly.  The level-(l+1) correction is in xb when that
+// level ran in the single workgroup (l == nmb), else in rb.  The result goes
+// to the level's residual slot (free by then), where the next finer level,
+// and for level 1 the particle level (e1), read it.
+__device__ __forceinline__ void mg_lvl_up(const MgDev& G, int l, int I) {
+  if (*G.done || l > G.nmb) return;
+  const MgLevel L = G.lev[l], C = G.lev[l + 1];
+  if (I >= L.n) return;
+  const float w = G.w, c = G.cscale;
+  const float* st = G.sten + 9 * L.off;
+  const float* b = G.bb + L.off;
+  const float* x = G.xb + L.off;
+  const float* __restrict__ xc = (l == G.nmb ? G.xb : G.rb) + C.off;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int N = mg_nbr(L, I, k);
+    if (N >= 0) {
+      const int nj = N / L.ny, ni = N - nj * L.ny;
+      acc = acc + st[k * L.n + I] * (x[N] + c * xc[(nj >> 1) * C.ny + (ni >> 1)]);
+    }
+  }
+  const int cj = I / L.ny, ci = I - cj * L.ny;
+  const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+  G.rb[L.off + I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+}
+
 __device__ __forceinline__ float mg_apply_row(const MgLevel& L, const float* st, const float* x,
                                               int I) {
   float acc = 0.0f;
@@ -266,13 +312,12 @@ __device__ __forceinline__ float mg_apply_row(const MgLevel& L, const float* st,
   return acc;
 }
 
-// levels l0..nlev in one workgroup (l0 = nmb + 1)
-__global__ void __launch_bounds__(MG_CO_THREADS) k_mg_coarse(MgDev G, int l0) {
+// levels nmb+1..nlev in one workgroup: down sweeps, dense solve, up sweeps
+__device__ __forceinline__ void mg_coarse_block(const MgDev& G) {
   if (*G.done) return;
   const int t = threadIdx.x, T = blockDim.x;
-  const int nl = G.nlev;
+  const int nl = G.nlev, l0 = G.nmb + 1;
   const float w = G.w, c = G.cscale;
-  // down: pre-smooth, residual, restrict
   for (int l = l0; l < nl; l++) {
     const MgLevel L = G.lev[l];
     const float* st = G.sten + 9 * L.off;
@@ -297,7 +342,6 @@ __global__ void __launch_bounds__(MG_CO_THREADS) k_mg_coarse(MgDev G, int l0) {
     }
     __syncthreads();
   }
-  // coarsest: explicit inverse
   {
     const MgLevel L = G.lev[nl];
     const float* b = G.bb + L.off;
@@ -309,7 +353,6 @@ __global__ void __launch_bounds__(MG_CO_THREADS) k_mg_coarse(MgDev G, int l0) {
     }
     __syncthreads();
   }
-  // up: prolong, correct, post-smooth
   for (int l = nl - 1; l >= l0; l--) {
     const MgLevel L = G.lev[l], C = G.lev[l + 1];
     const float* st = G.sten + 9 * L.off;
@@ -329,76 +372,37 @@ __global__ void __launch_bounds__(MG_CO_THREADS) k_mg_coarse(MgDev G, int l0) {
   }
 }
 
-// Cell level l (1 <= l < nlev) on many workgroups.  Down: one thread per
-// level-(l+1) aggregate pre-smooths its (up to four) cells, forms their
-// residuals with the neighbours' pre-smoothed values computed on the fly, and
-// restricts.  The arithmetic and its order are those of the single-workgroup
-// kernel below, so a level gives the same values whichever kernel runs it.
-__global__ void k_mg_lvl_down(MgDev G, int l) {
-  const int J = blockIdx.x * blockDim.x + threadIdx.x;
-  const MgLevel L = G.lev[l], C = G.lev[l + 1];
-  if (*G.done || J >= C.n) return;
-  const float w = G.w;
-  const float* st = G.sten + 9 * L.off;
-  const float* b = G.bb + L.off;
-  float* x = G.xb + L.off;
-  const int CJ = J / C.ny, CI = J - CJ * C.ny;
-  float s = 0.0f;
-  for (int bq = 0; bq < 2; bq++)
-    for (int a = 0; a < 2; a++) {
-      const int ci = 2 * CI + a, cj = 2 * CJ + bq;
-      if (ci >= L.ny || cj >= L.nx) continue;
-      const int I = cj * L.ny + ci;
-      float acc = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 9; k++) {
-        const int N = mg_nbr(L, I, k);
-        if (N >= 0) acc = acc + st[k * L.n + I] * (w * b[N] / st[4 * L.n + N]);
-      }
-      x[I] = w * b[I] / st[4 * L.n + I];
-      s = s + (b[I] - acc);
-    }
-  G.bb[C.off + J] = s;
+// level 0, row i: x = x1 + w (v - A x1) d with x1 = x0 + c P e1 (x1 of the
+// neighbours formed on the fly)
+template <typename VT>
+__device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
+                                             const VT* __restrict__ v, float* __restrict__ out,
+                                             int i) {
+  if (*G.done || i >= D.sc->n) return;
+  const float c = G.cscale;
+  const float* __restrict__ e1 = G.e1;
+  const float* __restrict__ x0 = G.x0;
+  const int32_t* __restrict__ cell = D.cell;
+  const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
+                                              [&](int j) { return x0[j] + c * e1[cell[j]]; });
+  const float x1 = G.x0[i] + c * e1[D.cell[i]];
+  out[i] = x1 + G.w * ((float)v[i] - acc) * G.d0[i];
 }
 
-// Up: one thread per cell of level l adds the prolonged level-(l+1)
-// correction xc and post-smooths, with the neighbours' corrected values
-// formed on the fly.  The result goes to the level's residual slot (free by
-// then), which is where the next finer level, and for level 1 the particle
-// level (e1), read it.
-__global__ void k_mg_lvl_up(MgDev G, int l, const float* __restrict__ xc) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  const MgLevel L = G.lev[l], C = G.lev[l + 1];
-  if (*G.done || I >= L.n) return;
-  const float w = G.w, c = G.cscale;
-  const float* st = G.sten + 9 * L.off;
-  const float* b = G.bb + L.off;
-  const float* x = G.xb + L.off;
-  float acc = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int N = mg_nbr(L, I, k);
-    if (N >= 0) {
-      const int nj = N / L.ny, ni = N - nj * L.ny;
-      acc = acc + st[k * L.n + I] * (x[N] + c * xc[(nj >> 1) * C.ny + (ni >> 1)]);
-    }
-  }
-  const int cj = I / L.ny, ci = I - cj * L.ny;
-  const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
-  G.rb[L.off + I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
-}
-
-// ---- preconditioned mixed-precision BiCGStab ----
-// ph = M(p), t1 = A32 ph ; partial (t1 . rh)
-__global__ void __launch_bounds__(256) k_pb_1(Dev D, MgDev G, const float* __restrict__ ph) {
-  __shared__ double lds[RED_BLOCK];
+// ---------------------------------------------------------------------
+// preconditioned mixed-precision BiCGStab, row bodies (row block vb).
+// ph = M(p) lives in r32, sh = M(s) in sv32.  Blocks write partial dot
+// products at vb and nblk + vb.  Every early return below depends only on
+// the scenario's scalars, so it is taken by whole workgroups.
+// ---------------------------------------------------------------------
+// t1 = A32 ph ; partial (t1 . rh)
+__device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
   if (D.sc->done) return;
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   double prod = 0.0;
   if (i < n) {
+    const float* __restrict__ ph = D.r32;
     const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
                                                 [&](int j) { return ph[j]; });
     D.t132[i] = acc;
@@ -408,27 +412,22 @@ __global__ void __launch_bounds__(256) k_pb_1(Dev D, MgDev G, const float* __res
   if (threadIdx.x == 0) D.part[vb] = s;
 }
 
-// s = r - alpha t1 (FP64 and an FP32 copy for the V-cycle)
-__global__ void k_pb_s(Dev D) {
+// s = r - alpha t1 (FP64)
+__device__ __forceinline__ void pb_s_rows(const Dev& D, int vb) {
   if (D.sc->done) return;
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
-  const double s = D.r[i] - D.sc->alpha * (double)D.t132[i];
-  D.sv[i] = s;
+  D.sv[i] = D.r[i] - D.sc->alpha * (double)D.t132[i];
 }
 
 // t2 = A32 sh ; partials (t2 . s), (t2 . t2)
-__global__ void __launch_bounds__(256) k_pb_2(Dev D, MgDev G, const float* __restrict__ sh) {
-  __shared__ double lds[RED_BLOCK];
+__device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
   if (D.sc->done) return;
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
+    const float* __restrict__ sh = D.sv32;
     const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
                                                 [&](int j) { return sh[j]; });
     D.t232[i] = acc;
@@ -444,18 +443,14 @@ __global__ void __launch_bounds__(256) k_pb_2(Dev D, MgDev G, const float* __res
 }
 
 // x += alpha ph + omega sh (FP64) ; r = s - omega t2 ; partials (r . rh), (r . r)
-__global__ void __launch_bounds__(256) k_pb_3(Dev D, const float* __restrict__ ph,
-                                              const float* __restrict__ sh) {
-  __shared__ double lds[RED_BLOCK];
+__device__ __forceinline__ void pb_3_rows(const Dev& D, int vb, double* lds) {
   if (D.sc->done) return;
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = D.sc->n;
   const double alpha = D.sc->alpha, omega = D.sc->omega;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
-    D.sol[i] = (D.sol[i] + alpha * (double)ph[i]) + omega * (double)sh[i];
+    D.sol[i] = (D.sol[i] + alpha * (double)D.r32[i]) + omega * (double)D.sv32[i];
     const double r = D.sv[i] - omega * (double)D.t232[i];
     D.r[i] = r;
     p1 = r * (double)D.rh32[i];
@@ -470,23 +465,29 @@ __global__ void __launch_bounds__(256) k_pb_3(Dev D, const float* __restrict__ p
 }
 
 // p = r + beta (p - omega t1)  (p stored FP32)
-__global__ void k_pb_4(Dev D) {
+__device__ __forceinline__ void pb_4_rows(const Dev& D, int vb) {
   if (D.sc->done) return;
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
   const double beta = D.sc->beta, omega = D.sc->omega;
   D.pp32[i] = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * (double)D.t132[i]));
 }
 
-// r = b - A64 x (FP64 true residual); on init also rh32 = p32 = (float) r
-__global__ void __launch_bounds__(256) k_pb_resid(Dev D, int init) {
-  __shared__ double lds[RED_BLOCK];
-  const int vb = row_block();
-  if (vb >= D.nblk) return;
+// r = b - A64 x (FP64 true residual), and a fresh start of the Krylov space
+// from it: rh32 = p32 = (float) r; partials (r . rh), (r . r).
+// init = 1: start of the solve; a scenario whose right-hand side is zero
+//           (finished by the norm) gets x = 0 instead.
+// init = 2: check of a scenario whose iteration stopped (done, not final);
+//           if the check fails the iteration restarts from this state.
+__device__ __forceinline__ void pb_resid_rows(const Dev& D, int vb, int init, double* lds) {
+  const DevScalars* S = D.sc;
   const int i = vb * RED_BLOCK + threadIdx.x;
-  const int n = D.sc->n;
+  const int n = S->n;
+  if (init == 1 && S->fin) {
+    if (S->bnorm == 0.0 && i < n) D.sol[i] = 0.0;
+    return;
+  }
+  if (init == 2 && !(S->done && !S->fin)) return;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
     const double* __restrict__ sol = D.sol;
@@ -494,11 +495,9 @@ __global__ void __launch_bounds__(256) k_pb_resid(Dev D, int init) {
                                                   [&](int j) { return sol[j]; });
     const double r = D.b[i] - acc;
     D.r[i] = r;
-    if (init) {
-      D.rh32[i] = (float)r;
-      D.pp32[i] = (float)r;
-    }
-    p1 = r * (double)(init ? (float)r : D.rh32[i]);
+    D.rh32[i] = (float)r;
+    D.pp32[i] = (float)r;
+    p1 = r * (double)(float)r;
     p2 = r * r;
   }
   const double s1 = block_tree(p1, lds);
@@ -509,21 +508,58 @@ __global__ void __launch_bounds__(256) k_pb_resid(Dev D, int init) {
   }
 }
 
-enum { PFIN_ALPHA = 0, PFIN_OMEGA = 1, PFIN_BETA = 2, PFIN_RESID = 3 };
+// partial (b . b)
+__device__ __forceinline__ void pb_bnorm_rows(const Dev& D, int vb, double* lds) {
+  const int i = vb * RED_BLOCK + threadIdx.x;
+  const double prod = (i < D.sc->n) ? D.b[i] * D.b[i] : 0.0;
+  const double s = block_tree(prod, lds);
+  if (threadIdx.x == 0) D.part[vb] = s;
+}
 
-// one-block finalize for the preconditioned solve.  PFIN_BETA stops on the
-// recursive residual only as a candidate: the host then checks the FP64
-// residual (PFIN_RESID) and continues if it is not met.
-__global__ void __launch_bounds__(256) k_pfinalize(Dev D, int op, int nb, int init) {
-  __shared__ double lds[RED_BLOCK];
+// ---------------------------------------------------------------------
+// one-workgroup finalize per scenario over its row blocks' partials.
+//   PFIN_BNORM  bnorm = sqrt(b . b); a zero or non-finite norm finishes the
+//               scenario (converged with x = 0 for a zero right-hand side)
+//   PFIN_ALPHA / PFIN_OMEGA / PFIN_BETA  the BiCGStab scalars; PFIN_BETA
+//               stops on the recursive residual at tol/4 only as a candidate
+//   PFIN_RESID  init 1: start state from the FP64 residual;
+//               init 2: check of a stopped iteration against the FP64
+//               residual: met -> final; else, unless the iteration or restart
+//               cap is reached, restart BiCGStab from the FP64 residual
+// ---------------------------------------------------------------------
+enum { PFIN_ALPHA = 0, PFIN_OMEGA = 1, PFIN_BETA = 2, PFIN_RESID = 3, PFIN_BNORM = 4 };
+
+__device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, double* lds) {
   DevScalars* S = D.sc;
   if ((op == PFIN_ALPHA || op == PFIN_OMEGA || op == PFIN_BETA) && S->done) return;
+  if (op == PFIN_RESID && init == 1 && S->fin) return;
+  if (op == PFIN_RESID && init == 2 && !(S->done && !S->fin)) return;
+  const int nb = D.nblk;
   const double s0 = final_tree(D.part, nb, lds);
   double s1 = 0.0;
-  if (op != PFIN_ALPHA) s1 = final_tree(D.part + nb, nb, lds);
+  if (op == PFIN_OMEGA || op == PFIN_BETA || op == PFIN_RESID)
+    s1 = final_tree(D.part + nb, nb, lds);
   if (threadIdx.x != 0) return;
   const DevParams& P = D.P;
   switch (op) {
+    case PFIN_BNORM: {
+      const double bn = sqrt(s0);
+      S->bnorm = bn;
+      S->iters = 0;
+      S->restarts = 0;
+      S->brk = 0;
+      S->done = 0;
+      S->conv = 0;
+      S->fin = 0;
+      S->rel = 0.0;
+      if (bn == 0.0 || !isfinite(bn)) {
+        S->fin = 1;
+        S->done = 1;
+        S->conv = (bn == 0.0) ? 1 : 0;
+        S->rel = (bn == 0.0) ? 0.0 : bn;
+      }
+      break;
+    }
     case PFIN_ALPHA:
       S->alpha = S->rho / s0;
       if (!isfinite(S->alpha)) { S->brk = 1; S->done = 1; }
@@ -538,7 +574,7 @@ __global__ void __launch_bounds__(256) k_pfinalize(Dev D, int op, int nb, int in
       S->rr = rr;
       S->rel = sqrt(rr) / S->bnorm;
       if (sqrt(rr) <= 0.25 * P.tol * S->bnorm) {
-        S->conv = 1;  // candidate: confirmed against the FP64 residual by the host
+        S->conv = 1;  // candidate: confirmed against the FP64 residual (PFIN_RESID, init 2)
         S->done = 1;
       } else if (rhon == 0.0 || !isfinite(rhon)) {
         S->brk = 1;
@@ -550,15 +586,27 @@ __global__ void __launch_bounds__(256) k_pfinalize(Dev D, int op, int nb, int in
       }
       break;
     }
-    case PFIN_RESID:
-      // s0 = r . rh (rh = r on init), s1 = r . r
+    case PFIN_RESID: {
+      // s0 = r . rh with rh = (float) r, s1 = r . r
       S->rr = s1;
       S->rel = sqrt(s1) / S->bnorm;
       S->rho = s0;
       S->conv = (sqrt(s1) <= P.tol * S->bnorm) ? 1 : 0;
-      S->done = S->conv;
       S->brk = 0;
-      if (init == 1) S->iters = 0;
+      if (init == 1) {
+        S->iters = 0;
+        S->done = S->conv;
+        S->fin = S->conv;
+      } else if (S->conv) {
+        S->fin = 1;
+      } else if (S->iters >= P.max_iter) {
+        S->fin = 1;
+      } else {
+        S->restarts = S->restarts + 1;
+        if (S->restarts > P.max_outer) S->fin = 1;
+        else S->done = 0;
+      }
       break;
+    }
   }
 }

@@ -204,10 +204,19 @@ int isph_load_system(isph_ctx *ctx, int32_t n, const int32_t *row_len,
                      const int32_t *col, const double *val, const double *rhs,
                      const double *x0);
 
-/* ---- batched form: many independent scenarios in one context set ---- */
+/* ---- batched form: many independent scenarios stepped together ----
+ * No counterpart in the reference (a single-scenario program); this is the
+ * batched configuration's entry.  The contexts keep their own state; while
+ * they belong to a batch they share one stream, and the PPE solves of the
+ * precision-form contexts run as one set of launches over all of them.  A
+ * scenario's results are the same as when it is stepped alone.  Destroy the
+ * batch before any of its contexts.  isph_batch_step returns the worst
+ * non-fatal solve code (ISPH_ENOCONV) over the scenarios and steps;
+ * isph_batch_stats gives scenario s's last solve. */
 typedef struct isph_batch isph_batch;
 int isph_batch_create(int32_t n_scen, isph_ctx *const *ctxs, isph_batch **out);
 int isph_batch_step(isph_batch *b, int32_t nsteps);
+int isph_batch_stats(isph_batch *b, int32_t s, isph_solve_stats *out);
 int isph_batch_destroy(isph_batch *b);
 
 /* ---- kernel timing (HIP events on the context's stream) ----
