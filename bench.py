@@ -24,7 +24,17 @@ profiles/traffic_flume_650k.md, when that file is present.  The CPU baseline
 is the oracle (a C/OpenMP restatement of the reference) on a bounded sample
 of the same workload.
 
+Two more legs run by default, at bounded sizes, and go into the same line:
+  batched  64 independent flumes, N ~ U[20k, 300k] (about 10M particles),
+           precision form, stepped together (one set of solver launches for
+           all), against the same scenarios stepped one after another alone;
+  served   a seeded Poisson stream (2 requests/s) of flumes with N
+           log-uniform on [10k, 1M], admitted against a 240 GB HBM budget and
+           continuously batched; p50/p99 completion latency, scenarios/h and
+           particle-steps/s.  Steps per request are reduced (default 40).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode both|plain|chosen]
+                       [--no-batch] [--no-serve] [--no-cpu]
        python bench.py --copy-calibration   (a 1 GiB device copy, for the
                                              counter calibration only)
 """
@@ -90,6 +100,70 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
     )
 
 
+def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int) -> dict:
+    """The batched configuration: n_scen independent flumes, N ~ U[20k, 300k]
+    (seeded), precision form at 1e-8, stepped together; the same scenarios
+    stepped one after another alone give the comparison."""
+    mix = cases.scenario_mix(n_scen=n_scen, seed=2024, n_min=20_000, n_max=300_000)
+    for c in mix:
+        c.params.update(precision=1, tol=1e-8, max_outer=20)
+    scen = [pkg.Scenario(c) for c in mix]
+    for sc in scen:
+        sc.init_state()
+    b = pkg.Batch(scen)
+    if warmup:
+        b.step(warmup)
+    t0 = time.perf_counter()
+    b.step(steps)
+    wall = time.perf_counter() - t0
+    its = [b.stats(q).iterations for q in range(len(scen))]
+    b.close()
+    n_live = [sc.n for sc in scen]
+    for sc in scen:
+        sc.close()
+    alone = 0.0
+    for c in mix:
+        sc = pkg.Scenario(c)
+        sc.init_state()
+        if warmup:
+            sc.step(warmup)
+        t1 = time.perf_counter()
+        sc.step(steps)
+        alone += time.perf_counter() - t1
+        sc.close()
+    ps = sum(n_live) * steps / wall
+    ps_alone = sum(n_live) * steps / alone
+    return dict(scenarios=n_scen, particles=int(sum(n_live)), steps=steps, warmup=warmup,
+                seconds=wall, particle_steps_per_s=ps, alone_particle_steps_per_s=ps_alone,
+                speedup_vs_alone=ps / ps_alone,
+                scenarios_per_h_at_1000_steps=3600.0 * n_scen / (1000.0 * wall / steps),
+                iterations_last_step_mean=float(np.mean(its)),
+                iterations_last_step_max=int(max(its)))
+
+
+def run_serve(pkg, cases, n_req: int, steps: int, rate: float) -> dict:
+    """The served configuration: a seeded Poisson stream of flume requests,
+    N log-uniform on [10k, 1M], admitted against a 240 GB HBM budget, each
+    kept device resident for its steps (reduced from the configuration's
+    2000 to bound the run), continuous batching between steps."""
+    from beach_slope_amd import serve
+    mix = cases.scenario_mix(n_scen=n_req, seed=99, n_min=10_000, n_max=1_000_000,
+                             log_uniform=True, steps=steps)
+    for c in mix:
+        c.params.update(precision=1, tol=1e-8, max_outer=20)
+    arr = serve.poisson_arrivals(n_req, rate=rate, seed=99)
+    reqs = [serve.Request(k, c, steps=steps, arrival=float(arr[k])) for k, c in enumerate(mix)]
+    srv = serve.Server(pkg, hbm_budget=240 * 10**9, max_slots=64, clock=serve.WallClock(),
+                       keep_results=False)
+    srv.run(reqs)
+    wall = srv.clock.now()
+    out = serve.summary(reqs, wall, srv.particle_steps)
+    out.update(rate_per_s=rate, steps_per_request=steps, hbm_budget_gb=240,
+               particles_total=int(sum(c.n for c in mix)), batched_steps=srv.steps_run,
+               gpu_busy_s=srv.busy)
+    return out
+
+
 KERNEL_OF = {"spmv_s": "k_bicg_2", "spmv_s32": "k_pb_2"}
 TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_flume_650k.md"
 
@@ -144,6 +218,12 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-scenarios", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-steps", type=int, default=40)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -165,6 +245,13 @@ def main() -> int:
         res["chosen"] = run_gpu(pkg, c, 1, args.steps, args.warmup, args.tol)
         print(f"bench: chosen {res['chosen']}", file=sys.stderr, flush=True)
     head = res.get("chosen") or res["plain"]
+    batched = served = None
+    if not args.no_batch:
+        batched = run_batch(pkg, cases, args.batch_scenarios, args.batch_steps, 1)
+        print(f"bench: batched {batched}", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        served = run_serve(pkg, cases, args.serve_requests, args.serve_steps, 2.0)
+        print(f"bench: served {served}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(entry.load_oracle(), c, args.tol)
@@ -191,6 +278,8 @@ def main() -> int:
                      "traffic": measured_traffic(head["kernel"]), "kernel": head["kernel"],
                      "avg_launch_us": head["kernel_avg_us"]},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     for k, v in res.items():
         out[k] = {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
