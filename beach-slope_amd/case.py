@@ -159,7 +159,9 @@ def dam_break(dx: float = 0.005, jitter: float = 0.0, seed: int = 1234,
     _grid(params, off + L + 4 * dx, off + H + 4 * dx)
     name = "dam_break_jitter" if jitter else "dam_break"
     return _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, off + column[1],
-                     steps=steps, meta=dict(dx=dx, jitter=jitter, seed=seed))
+                     steps=steps, meta=dict(dx=dx, jitter=jitter, seed=seed, off=off,
+                                            toe=off + toe, slope=slope, tank=tank,
+                                            column=column))
 
 
 def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,

@@ -974,13 +974,14 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   return 0;
 }
 
-// run BiCGStab chunks until the device reports done
+// run BiCGStab chunks until the device reports done; on a breakdown below
+// the iteration cap restart from the true residual, at most max_outer times
 static int bicg_loop(isph_ctx* c, int* iters_out, int* brk_out, int* conv_out,
-                     double* rel_out) {
+                     double* rel_out, int* restarts_out) {
   const int n = c->n_hi;
-  int chunk = 4;
+  int chunk = 4, restarts = 0, prev = 0;
   DevScalars s;
-  for (int it = 0;; ) {
+  for (int64_t it = 0;; ) {
     for (int k = 0; k < chunk; k++) {
       TIMED_LAUNCH(0, k_bicg_1, n, c->D);
       FINALIZE(FIN_ALPHA, n);
@@ -993,15 +994,30 @@ static int bicg_loop(isph_ctx* c, int* iters_out, int* brk_out, int* conv_out,
     it += chunk;
     int rc = read_scalars(c, &s);
     if (rc) return rc;
-    if (s.done) break;
+    // the timed launches that did work: the chunk's leading iterations
+    if (c->prof) timers_collect(c, s.iters - prev);
+    prev = s.iters;
+    if (s.done) {
+      if (s.brk && !s.conv && s.iters < c->P.max_iter && restarts < c->P.max_outer) {
+        restarts++;
+        RLAUNCH(k_bicg_init, n, c->D);
+        FINALIZE(FIN_RESTART, n);
+        rc = read_scalars(c, &s);
+        if (rc) return rc;
+        if (s.done) break;
+        chunk = 4;
+        continue;
+      }
+      break;
+    }
     if (chunk < 32) chunk *= 2;
-    if (it > c->P.max_iter + 64) break;
+    if (it > (int64_t)(c->P.max_iter + 64) * (c->P.max_outer + 1)) break;
   }
-  if (c->prof) timers_collect(c, s.iters);
   *iters_out = s.iters;
   *brk_out = s.brk;
   *conv_out = s.conv;
   *rel_out = s.rel;
+  *restarts_out = restarts;
   return 0;
 }
 
@@ -1047,13 +1063,14 @@ static int ppe_solve_impl(isph_ctx* c, isph_solve_stats* stats) {
       DevScalars s;
       int rc = read_scalars(c, &s);
       if (rc) return rc;
-      int it = 0, brk = 0, conv = s.conv;
+      int it = 0, brk = 0, conv = s.conv, restarts = 0;
       double rel = s.rel;
       if (!s.done) {
-        rc = bicg_loop(c, &it, &brk, &conv, &rel);
+        rc = bicg_loop(c, &it, &brk, &conv, &rel, &restarts);
         if (rc) return rc;
       }
       st.iterations = it;
+      st.outer = restarts;
       st.breakdown = brk;
       st.converged = conv;
       st.rel_residual = rel;

@@ -2,6 +2,9 @@
 // order of the reference's bicgstab (solver.f90:182-213) with y = A x in the
 // row order of spmv2 (solver.f90:84-127).
 //
+// On a breakdown below the iteration cap the host restarts from the true
+// residual (k_bicg_init, FIN_RESTART), at most max_outer times.
+//
 // Each BiCGStab iteration is four row kernels, three of which write per-block
 // partial dot products (block_tree over 256 rows, in row-block order whatever
 // workgroup computes the block); a one-block finalize kernel then sums the
@@ -151,11 +154,12 @@ __global__ void k_copy_sol_to_p(Dev D) {
 // preceding row kernel
 // ---------------------------------------------------------------------
 enum {
-  FIN_INIT = 0,   // rho = rr = sum ; convergence test
-  FIN_ALPHA = 1,  // alpha = rho / sum
-  FIN_OMEGA = 2,  // omega = sum0 / sum1
-  FIN_BETA = 3,   // rho' = sum0, rr = sum1 ; iteration bookkeeping, beta
-  FIN_DOT = 4     // *out = sum
+  FIN_INIT = 0,     // rho = rr = sum ; convergence test
+  FIN_ALPHA = 1,    // alpha = rho / sum
+  FIN_OMEGA = 2,    // omega = sum0 / sum1
+  FIN_BETA = 3,     // rho' = sum0, rr = sum1 ; iteration bookkeeping, beta
+  FIN_DOT = 4,      // *out = sum
+  FIN_RESTART = 5   // as FIN_INIT after a breakdown, keeping the iteration count
 };
 
 __global__ void __launch_bounds__(256) k_finalize(Dev D, int op, int nb, double* out) {
@@ -170,9 +174,10 @@ __global__ void __launch_bounds__(256) k_finalize(Dev D, int op, int nb, double*
   const DevParams& P = D.P;
   switch (op) {
     case FIN_INIT:
+    case FIN_RESTART:
       S->rho = s0;
       S->rr = s0;
-      S->iters = 0;
+      if (op == FIN_INIT) S->iters = 0;
       S->brk = 0;
       S->conv = 0;
       S->done = 0;

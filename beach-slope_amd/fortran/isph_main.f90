@@ -1,13 +1,22 @@
 ! isph_main.f90 -- Fortran driver of the device ISPH pipeline.
 !
-!   isph_driver <case.bin> <state_out.bin> [fused]
+!   isph_driver <case.bin> <state_out.bin> [fused] [snapshots=DIR]
 !
 ! Reads a case (run parameters, particles, porous solids, end time and
 ! display interval) from a binary stream file, runs the production loop of
 ! the reference (main.f90:199-261) through the C ABI, one call per reference
 ! subroutine in the reference's order, prints one line per display interval
-! and writes the final particle state to <state_out.bin>.  With the third
-! argument "fused" each iteration is one isph_step call instead.
+! and writes the final particle state to <state_out.bin>.  With the argument
+! "fused" each iteration is one isph_step call instead.
+!
+! With snapshots=DIR the driver writes the reference's output files under DIR
+! (whose subdirectories fluid/, buffer/, boundary/ and ghost/ must exist, as
+! the reference expects): boundary/boundary_1000.txt, ghost/ghost_1000.txt
+! and porous.txt once at the start (output.f90:11-136), and at every display
+! time and after the loop fluid/fluid_<1000+iter>.txt and
+! buffer/buffer_<1000+iter>.txt (output.f90:188-222, main.f90:263-274), with
+! the same edit descriptors.  The library keeps particles in the reference's
+! cell order, so the lines come out in the order the reference writes them.
 !
 ! Case file (stream access, native endianness):
 !   int32 magic 'ISPH' (0x48505349), int32 version 1
@@ -41,6 +50,9 @@ program isph_main
   integer(c_int8_t), allocatable :: flags(:)
   logical :: fused
   integer(c_int64_t) :: nsteps
+  character(len=4096) :: snapdir, arg
+  logical :: snaps
+  integer :: ia
 
   if (command_argument_count() < 2) then
     write(*, '(a)') 'usage: isph_driver <case.bin> <state_out.bin> [fused]'
@@ -48,9 +60,22 @@ program isph_main
   end if
   call get_command_argument(1, inpath)
   call get_command_argument(2, outpath)
+  fused = .false.
+  snaps = .false.
+  snapdir = ''
+  do ia = 3, command_argument_count()
+    call get_command_argument(ia, arg)
+    if (trim(arg) == 'fused') then
+      fused = .true.
+    else if (arg(1:10) == 'snapshots=') then
+      snaps = .true.
+      snapdir = arg(11:)
+    else
+      write(*, '(2a)') 'unknown argument ', trim(arg)
+      stop 2
+    end if
+  end do
   mode = ''
-  if (command_argument_count() >= 3) call get_command_argument(3, mode)
-  fused = trim(mode) == 'fused'
 
   open(newunit=u, file=trim(inpath), access='stream', form='unformatted', status='old', &
        action='read', iostat=ios)
@@ -83,6 +108,12 @@ program isph_main
              'isph_upload')
   if (ns > 0) call check(isph_upload_solids(ctx, ns, sx, sy, sm, srho), 'isph_upload_solids')
   call check(isph_init_state(ctx), 'isph_init_state')
+
+  if (snaps) then
+    call print_walls(1_c_int32_t, 'boundary/boundary_')
+    call print_walls(2_c_int32_t, 'ghost/ghost_')
+    call print_porous()
+  end if
 
   write(*, '(a)') '#        t            dt      solsteps  iterations    rel_residual        n'
   iter = 1
@@ -124,10 +155,12 @@ program isph_main
       write(*, '(es16.8, es14.6, i8, i12, es16.6, i10)') sc%t, sc%dt, sc%solsteps, &
           st%iterations, st%rel_residual, sc%n
       flush(output_unit)
+      if (snaps) call combined(iter)
       iter = iter + 1
     end if
   end do
   write(*, '(a, i0, a)') '# ', nsteps, ' steps'
+  if (snaps) call combined(iter)
 
   call check(isph_download(ctx, nout, x, y, vx, vy, p, m, rho, rho0, vol0, con, tid, flags), &
              'isph_download')
@@ -141,6 +174,60 @@ program isph_main
   call check(isph_destroy(ctx), 'isph_destroy')
 
 contains
+
+  ! fluid/fluid_<1000+it>.txt and buffer/buffer_<1000+it>.txt (output.f90:188-222):
+  ! x, y, pressure, vx, vy and the concentration con + 0.5 of every fluid
+  ! particle, buffer particles in the second file
+  subroutine combined(it)
+    integer, intent(in) :: it
+    integer :: uf, ub, k
+    character(len=4200) :: f1, f2
+    call check(isph_download(ctx, nout, x, y, vx, vy, p, m, rho, rho0, vol0, con, tid, flags), &
+               'isph_download')
+    write(f1, '(a, "/fluid/fluid_", i0, ".txt")') trim(snapdir), 1000 + it
+    write(f2, '(a, "/buffer/buffer_", i0, ".txt")') trim(snapdir), 1000 + it
+    open(newunit=uf, file=trim(f1), status='replace', action='write')
+    open(newunit=ub, file=trim(f2), status='replace', action='write')
+    do k = 1, int(nout)
+      if (tid(k) /= ISPH_TID_FLUID) cycle
+      if (iand(flags(k), ISPH_F_BUFFER) == 0) then
+        write(uf, '(F10.3,1X,F10.3,1X,F10.3,1X,ES10.3,1X,ES10.3,1X,F10.5)') x(k), y(k), p(k), &
+            vx(k), vy(k), con(k) + 0.50_c_double
+      else
+        write(ub, '(F10.3,1X,F10.3,1X,F10.3,1X,ES10.3,1X,ES10.3,1X,F10.5)') x(k), y(k), p(k), &
+            vx(k), vy(k), con(k) + 0.50_c_double
+      end if
+    end do
+    close(uf)
+    close(ub)
+  end subroutine combined
+
+  ! boundary/boundary_1000.txt (tid 1) or ghost/ghost_1000.txt (tid 2):
+  ! x, y, pressure (output.f90:11-61)
+  subroutine print_walls(which, stem)
+    integer(c_int32_t), intent(in) :: which
+    character(len=*), intent(in) :: stem
+    integer :: uf, k
+    character(len=4200) :: f1
+    call check(isph_download(ctx, nout, x, y, vx, vy, p, m, rho, rho0, vol0, con, tid, flags), &
+               'isph_download')
+    write(f1, '(a, "/", a, i0, ".txt")') trim(snapdir), stem, 1000
+    open(newunit=uf, file=trim(f1), status='replace', action='write')
+    do k = 1, int(nout)
+      if (tid(k) == which) write(uf, '(F10.3,1X,F10.3,1X,F10.3)') x(k), y(k), p(k)
+    end do
+    close(uf)
+  end subroutine print_walls
+
+  ! porous.txt: x, y of the porous solids (output.f90:114-136), in upload order
+  subroutine print_porous()
+    integer :: uf, k
+    open(newunit=uf, file=trim(snapdir)//'/porous.txt', status='replace', action='write')
+    do k = 1, int(ns)
+      write(uf, '(F10.3,1X,F10.3)') sx(k), sy(k)
+    end do
+    close(uf)
+  end subroutine print_porous
 
   subroutine check(rc, what)
     integer(c_int), intent(in) :: rc

@@ -24,7 +24,10 @@ profiles/traffic_flume_650k.md, when that file is present.  The CPU baseline
 is the oracle (a C/OpenMP restatement of the reference) on a bounded sample
 of the same workload.
 
-Two more legs run by default, at bounded sizes, and go into the same line:
+Three more legs run by default, at bounded sizes, and go into the same line:
+  precision_errors  the jittered dam break, 200 steps in each form: pressure,
+           free-surface, front and runup differences of the precision form
+           against the FP64 run (gauges.py);
   batched  64 independent flumes, N ~ U[20k, 300k] (about 10M particles),
            precision form, stepped together (one set of solver launches for
            all), against the same scenarios stepped one after another alone;
@@ -164,6 +167,55 @@ def run_serve(pkg, cases, n_req: int, steps: int, rate: float) -> dict:
     return out
 
 
+def run_precision_errors(pkg, cases, steps: int) -> dict:
+    """The precision-form configuration: the dam break with seeded +-0.01 dx
+    jitter run `steps` steps in the FP64 form (tolerance 1e-10) and in the
+    precision form (1e-8) from the same state; pressure, free-surface, front
+    and runup differences through the gauges."""
+    from beach_slope_amd import gauges
+    c = cases.dam_break(jitter=0.01)
+    m = c.meta
+    h = c.params["h"]
+    out = {"workload": f"dam_break_jitter n={c.n}", "steps": steps}
+    states = {}
+    for form, prec, tol in (("fp64", 0, 1e-10), ("precision", 1, 1e-8)):
+        prm = dict(c.params, precision=prec, tol=tol, max_outer=20)
+        sc = pkg.Scenario(c, params=prm)
+        sc.init_state()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            _, st = sc.step(1)
+            its.append(st.iterations)
+        out[f"{form}_seconds"] = time.perf_counter() - t0
+        out[f"{form}_iterations_mean"] = float(np.mean(its))
+        states[form] = sc.download()
+        sc.close()
+    a, b = states["precision"], states["fp64"]
+    off, col_h = m["off"], m["column"][1]
+    probes = np.array([[off + 0.05, off + 0.02], [off + 0.25, off + 0.02],
+                       [off + 0.05, off + 0.12], [off + 0.25, off + 0.10]])
+    pa, pb = gauges.probe_pressure(a, probes, h), gauges.probe_pressure(b, probes, h)
+    stations = off + np.array([0.05, 0.15, 0.25, 0.35, 0.45])
+    ea = gauges.surface_elevation(a, stations, 0.5 * m["dx"])
+    eb = gauges.surface_elevation(b, stations, 0.5 * m["dx"])
+    fa, fb = gauges.front_position(a), gauges.front_position(b)
+    ra, rb = gauges.runup(a, m["toe"], off), gauges.runup(b, m["toe"], off)
+    pair = gauges.pair_states(a, b)
+    out.update(
+        probe_pressure_rel_l2=float(np.linalg.norm(pa - pb) / np.linalg.norm(pb)),
+        surface_max_abs_m=float(np.nanmax(np.abs(ea - eb))),
+        surface_max_rel_to_column=float(np.nanmax(np.abs(ea - eb)) / col_h),
+        front_abs_m=abs(fa - fb), front_m=fb,
+        runup_abs_m=abs(ra - rb), runup_m=rb,
+        paired_pressure_rel_l2=pair["pressure_rel_l2"],
+        paired_velocity_rel_l2=pair["velocity_rel_l2"],
+        paired_max_distance_m=pair["max_pair_distance"],
+        free_flag_mismatch=pair["free_flag_mismatch"],
+    )
+    return out
+
+
 KERNEL_OF = {"spmv_s": "k_bicg_2", "spmv_s32": "k_pb_2"}
 TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_flume_650k.md"
 
@@ -218,6 +270,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-scenarios", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=2)
@@ -245,7 +299,10 @@ def main() -> int:
         res["chosen"] = run_gpu(pkg, c, 1, args.steps, args.warmup, args.tol)
         print(f"bench: chosen {res['chosen']}", file=sys.stderr, flush=True)
     head = res.get("chosen") or res["plain"]
-    batched = served = None
+    batched = served = errors = None
+    if not args.no_errors:
+        errors = run_precision_errors(pkg, cases, args.error_steps)
+        print(f"bench: precision errors {errors}", file=sys.stderr, flush=True)
     if not args.no_batch:
         batched = run_batch(pkg, cases, args.batch_scenarios, args.batch_steps, 1)
         print(f"bench: batched {batched}", file=sys.stderr, flush=True)
@@ -278,6 +335,7 @@ def main() -> int:
                      "traffic": measured_traffic(head["kernel"]), "kernel": head["kernel"],
                      "avg_launch_us": head["kernel_avg_us"]},
         "cpu_baseline": cpu,
+        "precision_errors": errors,
         "batched": batched,
         "served": served,
     }

@@ -830,42 +830,59 @@ static void spmv(const ostate *s, const double *xv, double *yv) {
 }
 
 /* FP64 BiCGStab in the reference's update order (solver.f90:182-213). */
+/* BiCGStab in the update order of solver.f90:182-213, stopping on the
+ * relative residual.  On a breakdown (rho, omega or alpha zero or not
+ * finite) below the iteration cap it restarts from the true residual
+ * (r = b - A x, rh = p = r), at most max_outer times; iterations keep
+ * counting across restarts and st->outer counts the restarts.  (The
+ * reference's bicgstab has neither the residual stop nor the restart.) */
 static int bicgstab64(ostate *s, double *xv, const double *bv, double tol, int maxit,
-                      isph_solve_stats *st, double bnorm) {
+                      int max_outer, isph_solve_stats *st, double bnorm) {
   const int64_t n = s->n;
   double *r = s->w1, *rh = s->w2, *p = s->w3, *t1 = s->w4, *sv = s->w5, *t2 = s->w6, *tmp = s->w7;
-  spmv(s, xv, t1);
-  for (int64_t i = 0; i < n; i++) { r[i] = bv[i] - t1[i]; rh[i] = r[i]; p[i] = r[i]; }
-  double rho = dot(s, n, rh, r, tmp);
-  double rr = dot(s, n, r, r, tmp);
   st->iterations = 0;
   st->breakdown = 0;
   st->converged = 0;
-  if (sqrt(rr) <= tol * bnorm) { st->converged = 1; st->rel_residual = sqrt(rr) / bnorm; return 0; }
-  for (int it = 1; it <= maxit; it++) {
-    spmv(s, p, t1);
-    double alpha = rho / dot(s, n, t1, rh, tmp);
-    for (int64_t i = 0; i < n; i++) sv[i] = r[i] - alpha * t1[i];
-    spmv(s, sv, t2);
-    double omega = dot(s, n, t2, sv, tmp) / dot(s, n, t2, t2, tmp);
-    for (int64_t i = 0; i < n; i++) {
-      xv[i] = (xv[i] + alpha * p[i]) + omega * sv[i];
-      r[i] = sv[i] - omega * t2[i];
-    }
-    double rhon = dot(s, n, r, rh, tmp);
-    rr = dot(s, n, r, r, tmp);
-    st->iterations = it;
+  st->outer = 0;
+  int it = 0;
+  for (int restart = 0;; restart++) {
+    spmv(s, xv, t1);
+    for (int64_t i = 0; i < n; i++) { r[i] = bv[i] - t1[i]; rh[i] = r[i]; p[i] = r[i]; }
+    double rho = dot(s, n, rh, r, tmp);
+    double rr = dot(s, n, r, r, tmp);
     st->rel_residual = sqrt(rr) / bnorm;
+    st->breakdown = 0;
     if (sqrt(rr) <= tol * bnorm) { st->converged = 1; return 0; }
-    if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 || !isfinite(rhon)) {
-      st->breakdown = 1;
-      return 0;
+    int brk = 0;
+    while (it < maxit) {
+      it++;
+      spmv(s, p, t1);
+      double alpha = rho / dot(s, n, t1, rh, tmp);
+      for (int64_t i = 0; i < n; i++) sv[i] = r[i] - alpha * t1[i];
+      spmv(s, sv, t2);
+      double omega = dot(s, n, t2, sv, tmp) / dot(s, n, t2, t2, tmp);
+      for (int64_t i = 0; i < n; i++) {
+        xv[i] = (xv[i] + alpha * p[i]) + omega * sv[i];
+        r[i] = sv[i] - omega * t2[i];
+      }
+      double rhon = dot(s, n, r, rh, tmp);
+      rr = dot(s, n, r, r, tmp);
+      st->iterations = it;
+      st->rel_residual = sqrt(rr) / bnorm;
+      if (sqrt(rr) <= tol * bnorm) { st->converged = 1; return 0; }
+      if (!(isfinite(alpha) && isfinite(omega)) || omega == 0.0 || rhon == 0.0 || !isfinite(rhon)) {
+        brk = 1;
+        break;
+      }
+      double beta = (rhon / rho) * (alpha / omega);
+      for (int64_t i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * t1[i]);
+      rho = rhon;
     }
-    double beta = (rhon / rho) * (alpha / omega);
-    for (int64_t i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * t1[i]);
-    rho = rhon;
+    if (!brk) return 0;            /* iteration cap */
+    st->breakdown = 1;
+    if (it >= maxit || restart >= max_outer) return 0;
+    st->outer = restart + 1;
   }
-  return 0;
 }
 
 int or_ppe_solve(ostate *s, isph_solve_stats *out) {
@@ -885,7 +902,7 @@ int or_ppe_solve(ostate *s, isph_solve_stats *out) {
   } else {
     /* The oracle is the FP64 reference for both solve forms: the precision
      * form of the GPU path is checked against it by tolerance. */
-    bicgstab64(s, s->sol, s->b, s->P.tol, s->P.max_iter, &st, bnorm);
+    bicgstab64(s, s->sol, s->b, s->P.tol, s->P.max_iter, s->P.max_outer, &st, bnorm);
   }
   /* pressure scatter, isph.f90:468-486 */
   for (int64_t i = 0; i < n; i++) s->p[i] = s->sol[i];

@@ -97,3 +97,45 @@ def test_driver_matches_host_steps(pkg, cases, tmp_path, mode):
     assert got["t"] == sc.t and got["dt"] == sc.dt
     for key in caseio.PART_FIELDS + ("tid", "flags"):
         assert np.array_equal(got[key], ref[key]), key
+
+
+@pytest.mark.gpu
+def test_driver_snapshot_files(pkg, cases, tmp_path):
+    """snapshots=DIR: the reference's output files (output.f90:11-222) with
+    its names (1000 + iter), edit descriptors and line order; the last fluid
+    file holds the final state to the printed precision."""
+    _need_driver()
+    from beach_slope_amd import caseio
+    k = 2
+    c = cases.tiny_box(nx=10, ny=8, solids=True, patch=True)
+    dt = c.params["dt"]
+    f = tmp_path / "case.bin"
+    out = tmp_path / "state.bin"
+    snap = tmp_path / "snap"
+    for d in ("fluid", "buffer", "boundary", "ghost"):
+        (snap / d).mkdir(parents=True)
+    caseio.write_case(f, c, time=(k - 0.5) * dt, displaytime=dt)
+    print(f"driver snapshots: tiny_box n={c.n} steps={k}", flush=True)
+    r = subprocess.run([str(DRIVER), str(f), str(out), f"snapshots={snap}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    names = sorted(p.name for p in (snap / "fluid").iterdir())
+    assert names == [f"fluid_{1000 + i}.txt" for i in range(1, k + 2)], names
+    assert (snap / "boundary" / "boundary_1000.txt").exists()
+    assert (snap / "ghost" / "ghost_1000.txt").exists()
+    por = np.loadtxt(snap / "porous.txt", ndmin=2)
+    assert por.shape == (c.ns, 2)
+    nb = len((snap / "boundary" / "boundary_1000.txt").read_text().splitlines())
+    assert nb == int(np.sum(c.parts["tid"] == 1))
+    st = caseio.read_state(out)
+    lines = (snap / "fluid" / f"fluid_{1000 + k + 1}.txt").read_text().splitlines()
+    sel = (st["tid"] == 3) & ((st["flags"] & 1) == 0)
+    assert len(lines) == int(sel.sum())
+    assert all(len(ln) == 10 + 1 + 10 + 1 + 10 + 1 + 10 + 1 + 10 + 1 + 10 for ln in lines)
+    vals = np.array([[float(ln[i:i + 10]) for i in range(0, 66, 11)] for ln in lines])
+    want = np.stack([st["x"][sel], st["y"][sel], st["p"][sel], st["u"][sel], st["v"][sel],
+                     st["c"][sel] + 0.5], axis=1)
+    assert np.all(np.abs(vals[:, :3] - want[:, :3]) <= 5e-4 + 1e-12)
+    for col in (3, 4):
+        assert np.all(np.abs(vals[:, col] - want[:, col]) <= 5e-4 * np.abs(want[:, col]) + 1e-300)
+    assert np.all(np.abs(vals[:, 5] - want[:, 5]) <= 5e-6 + 1e-12)

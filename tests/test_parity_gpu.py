@@ -183,6 +183,30 @@ def test_fused_steps_bit_exact_dam_break(pkg, oracle, cases):
     compare_scalars(g, o, "after 5 fused steps")
 
 
+def test_fp64_breakdown_restart_bit_exact(pkg, oracle, cases):
+    """The jittered dam break's FP64 solve breaks down at step 24 (rho or
+    omega vanishes at a relative residual near 1e-8, above the 1e-10 target);
+    both paths restart from the true residual and converge, with the same
+    iterations, restarts and fields bit for bit."""
+    c = cases.dam_break(jitter=0.01)
+    g = pkg.Scenario(c)
+    o = oracle.Oracle(c)
+    g.init_state()
+    o.call("init_state")
+    restarts = 0
+    for k in range(26):
+        _, st = g.step(1)
+        rc, sto = o.step(1)
+        assert rc == 0, (k, rc)
+        assert (st.iterations, st.outer, st.converged) == \
+            (sto.iterations, sto.outer, sto.converged), k
+        restarts += st.outer
+    print(f"dam_break_jitter 26 steps: {restarts} breakdown restarts", flush=True)
+    assert restarts >= 1
+    compare_state(g, o, "after 26 fused steps with a breakdown restart")
+    compare_scalars(g, o, "after 26 fused steps with a breakdown restart")
+
+
 def test_precision_form_residual_and_pressure_error(pkg, oracle, cases):
     """Precision form (cell-multigrid preconditioned BiCGStab, FP32 matrix,
     V-cycles and directions, FP64 solution / residual / dots): every solve
