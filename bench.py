@@ -59,15 +59,6 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 650k ISPH; scenarios/h + p99 latency vs plain GPU FP64"
 
 
-def spmv_s_bytes(n: int, nnz: float, fp32: bool) -> float:
-    """Algorithmic bytes of one launch of t = A s (s = r - alpha t1 formed on
-    the fly): row lengths, column indices and values once, r and t1 once,
-    s and t written once."""
-    if fp32:
-        return n * (4 + 8 + 8) + nnz * (4 + 4)
-    return n * (4 + 16 + 16) + nnz * (4 + 8)
-
-
 def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
     params = dict(case.params)
     params.update(precision=precision, tol=tol, inner_tol=1e-3, max_outer=20)
@@ -92,7 +83,8 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
     g.close()
     wall = t1 - t0
     avg_us = 1e3 * ms / max(launches, 1)
-    bytes_per = spmv_s_bytes(n, nnz, bool(precision))
+    from beach_slope_amd import bytes_model
+    bytes_per = bytes_model.kernel_bytes(KERNEL_OF[kname], n, nnz)
     achieved = bytes_per / (avg_us * 1e-6) / 1e9 if launches else 0.0
     return dict(
         n=n, seconds=wall, ms_per_step=1e3 * wall / steps,
@@ -216,7 +208,7 @@ def run_precision_errors(pkg, cases, steps: int) -> dict:
     return out
 
 
-KERNEL_OF = {"spmv_s": "k_bicg_2", "spmv_s32": "k_pb_2"}
+KERNEL_OF = {"spmv_s": "k_bicg_2", "spmv_s32": "kb_pb_2"}
 TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_flume_650k.md"
 
 
