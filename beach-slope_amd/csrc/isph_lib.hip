@@ -244,10 +244,10 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   }
   c->G.done = &D.sc->done;
   if (params->precision) {
-    // multi-block cell levels while the next level exceeds half a workgroup
+    // multi-block cell levels while the next level exceeds MG_MB_MIN unknowns
     MgDev& G = c->G;
     G.nmb = 0;
-    while (G.nmb + 1 < G.nlev && G.lev[G.nmb + 2].n > MG_CO_THREADS / 2) G.nmb++;
+    while (G.nmb + 1 < G.nlev && G.lev[G.nmb + 2].n > MG_MB_MIN) G.nmb++;
     if (G.nlev >= 2 && G.nmb == 0) G.nmb = 1;
     // level 1's final correction: rb when level 1 runs multi-block, else xb
     G.e1 = (G.nmb >= 1) ? G.rb + G.lev[1].off : G.xb + G.lev[1].off;
@@ -261,8 +261,8 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
     isph_destroy(c);
     return create_fail(ISPH_EHIP, "isph_create: hipMemcpy failed: %s", hipGetErrorString(he));
   }
-  hipMemset(D.sstart, 0, sizeof(int32_t) * (c->ncell + 1));
-  hipMemset(D.vic, 0, N);
+  (void)hipMemset(D.sstart, 0, sizeof(int32_t) * (c->ncell + 1));
+  (void)hipMemset(D.vic, 0, N);
   *out = c;
   return 0;
 }
@@ -271,14 +271,14 @@ static void bh_free(BatchHost* b);
 
 extern "C" int isph_destroy(isph_ctx* c) {
   if (!c) return 0;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   bh_free(c->self_batch);
   for (void* p : c->allocs) (void)hipFree(p);
   for (KTimer& t : c->timers) {
     for (auto& pr : t.pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (auto& pr : t.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
   }
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }

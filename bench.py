@@ -61,7 +61,7 @@ METRIC = "particle-steps/s at 650k ISPH; scenarios/h + p99 latency vs plain GPU 
 
 def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
     params = dict(case.params)
-    params.update(precision=precision, tol=tol, inner_tol=1e-3, max_outer=20)
+    params.update(precision=precision, tol=tol, inner_tol=1.0, max_outer=20)
     g = pkg.Scenario(case, params=params)
     g.init_state()
     for _ in range(warmup):

@@ -99,7 +99,8 @@ typedef struct isph_params {
   double buf_xmax;
   double dt;              /* initial (or fixed) time step                      */
   double tol;             /* PPE relative residual target ||b-Ax||/||b||       */
-  double inner_tol;       /* reserved, unused (kept for layout stability)      */
+  double inner_tol;       /* precision form: the recursive residual stops at
+                             inner_tol * tol (0: 1.0) before the FP64 check   */
 } isph_params;
 
 /* Statistics of the last PPE solve. */
