@@ -234,8 +234,8 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
     rc |= dalloc(c, &G.rowst, 9 * N);
     rc |= dalloc(c, &G.d0, N);
     rc |= dalloc(c, &G.x0, N);
-    G.w = 0.7f;
-    G.cscale = 1.6f;
+    G.w = MG_OMEGA;
+    G.cscale = MG_CSCALE;
   }
   if (rc) {
     const std::string e = c->err;

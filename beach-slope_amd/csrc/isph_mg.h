@@ -34,6 +34,25 @@
 #ifndef MG_MB_MIN
 #define MG_MB_MIN (2 * MG_CO_THREADS)
 #endif
+// Recursive-residual target of the cycles after a restart, as a fraction of
+// tol (the first cycle's fraction is the parameter inner_tol).  The last
+// cycle sets the final residual, and with it how much smooth error is left:
+// on the jittered dam break's same-system check the pressure error is
+// 4.0e-6 at 1.0 and 2.1e-7 at 0.25 (against a 1e-13 FP64 solve).
+#ifndef PB_LATER_FRAC
+#define PB_LATER_FRAC 0.25
+#endif
+
+// Damped-Jacobi weight and coarse-correction scale of the V-cycle, from a
+// sweep at 650k particles (ms per step, BiCGStab iterations per solve):
+// 0.7/1.6 29.0 ms, 62; 0.8/1.9 25.6 ms, 55; 0.9/1.9 22.8 ms, 48;
+// 0.95/1.9 23.0 ms; 1.0/1.9 23.3 ms; 0.9/1.7 24.4 ms; 0.9/2.1 25.3 ms.
+#ifndef MG_OMEGA
+#define MG_OMEGA 0.9f
+#endif
+#ifndef MG_CSCALE
+#define MG_CSCALE 1.9f
+#endif
 
 struct MgLevel {
   int ny, nx, n;   // grid rows (ci), columns (cj), unknowns n = ny*nx
@@ -529,12 +548,14 @@ __device__ __forceinline__ void pb_bnorm_rows(const Dev& D, int vb, double* lds)
 //               scenario (converged with x = 0 for a zero right-hand side)
 //   PFIN_ALPHA / PFIN_OMEGA / PFIN_BETA  the BiCGStab scalars; PFIN_BETA
 //               stops on the recursive residual at frac * tol only as a
-//               candidate (frac = inner_tol, 1 when unset).  The FP32
-//               products keep the recursive residual from following the true
-//               one below about eps32 ||A|| ||x||, so the first check fails
-//               and one restart from the FP64 residual finishes the solve;
-//               stopping the first cycle at tol itself wastes the fewest
-//               iterations (at 650k: 62 per solve at 1, 68 at 0.25, 81 at 0.1)
+//               candidate (first cycle: frac = inner_tol, 1 when unset; later
+//               cycles: PB_LATER_FRAC).  The FP32 products keep the recursive
+//               residual from following the true one below about
+//               eps32 ||A|| ||x||, so the first check fails and a restart from
+//               the FP64 residual finishes the solve; stopping the first cycle
+//               at tol itself wastes the fewest iterations (at 650k with the
+//               earlier V-cycle constants: 62 per solve at 1, 68 at 0.25, 81
+//               at 0.1)
 //   PFIN_RESID  init 1: start state from the FP64 residual;
 //               init 2: check of a stopped iteration against the FP64
 //               residual: met -> final; else, unless the iteration or restart
@@ -586,7 +607,8 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, doub
       S->iters = S->iters + 1;
       S->rr = rr;
       S->rel = sqrt(rr) / S->bnorm;
-      const double frac = (P.inner_tol > 0.0) ? P.inner_tol : 1.0;
+      const double frac = (S->restarts == 0) ? ((P.inner_tol > 0.0) ? P.inner_tol : 1.0)
+                                             : PB_LATER_FRAC;
       if (sqrt(rr) <= frac * P.tol * S->bnorm) {
         S->conv = 1;  // candidate: confirmed against the FP64 residual (PFIN_RESID, init 2)
         S->done = 1;

@@ -234,4 +234,4 @@ def test_precision_form_residual_and_pressure_error(pkg, oracle, cases):
     rel = np.linalg.norm(x32 - x64) / np.linalg.norm(x64)
     print(f"precision form: {st.iterations} iterations, rel residual {st.rel_residual:.2e}; "
           f"FP64 reference {st64.iterations} iterations; pressure error {rel:.2e}")
-    assert rel <= 1e-5
+    assert rel <= 1e-6  # the precision form's pressure target
