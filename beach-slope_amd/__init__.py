@@ -82,7 +82,7 @@ EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_step", "isph_dump_cells", "isph_dump_neighbours", "isph_dump_props",
            "isph_dump_ppe", "isph_ppe_set_rhs", "isph_ppe_get_solution",
            "isph_load_system", "isph_batch_create", "isph_batch_step", "isph_batch_stats",
-           "isph_batch_destroy",
+           "isph_batch_destroy", "isph_set_buffer", "isph_buffer_update",
            "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read") + tuple("isph_" + p for p in PHASES)
 
 _lib = None
@@ -129,6 +129,8 @@ def lib():
     L.isph_batch_step.argtypes = [vp, i32]
     L.isph_batch_stats.argtypes = [vp, i32, C.POINTER(SolveStats)]
     L.isph_batch_destroy.argtypes = [vp]
+    L.isph_set_buffer.argtypes = [vp, i32, i64] + [vp] * 5 + [d] * 7
+    L.isph_buffer_update.argtypes = [vp, i32]
     L.isph_prof_enable.argtypes = [vp, i32]
     L.isph_prof_read.argtypes = [vp, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
     L.isph_footprint_bytes.restype = i64
@@ -304,6 +306,20 @@ class Scenario:
         self._check(lib().isph_dump_ppe(self._ctx, _ptr(ln), _ptr(col), _ptr(val), _ptr(rhs)),
                     "isph_dump_ppe")
         return ln, col, val, rhs
+
+    def set_buffer(self, zone: int, x, y, u, v, p, mass: float, rho: float, vol0: float,
+                   c: float, rv: float, interval: float = 0.0, time_shift: float = 0.0):
+        """Buffer zone 1 or 2 (isph_set_buffer): template particles and the
+        values inserted particles take, removal coordinate, insertion clock."""
+        t = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u, v, p)]
+        self._keep_buf = t
+        self._check(lib().isph_set_buffer(self._ctx, zone, t[0].size, *[_ptr(a) for a in t],
+                                          mass, rho, vol0, c, rv, interval, time_shift),
+                    "isph_set_buffer")
+
+    def buffer_update(self, zone: int):
+        """remove_buffer then insert_buffer of the zone (isph_buffer_update)."""
+        self._check(lib().isph_buffer_update(self._ctx, zone), "isph_buffer_update")
 
     def load_system(self, length, col, val, rhs, x0):
         length = np.ascontiguousarray(length, np.int32)

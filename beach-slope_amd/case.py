@@ -331,3 +331,42 @@ def tiny_box(nx: int = 12, ny: int = 10, dx: float = 0.01, jitter: float = 0.05,
     c.parts["u"][wx.size:] = 0.05 * np.sin(fy * 40.0)
     c.parts["v"][wx.size:] = 0.03 * np.cos(fx * 30.0)
     return c
+
+
+def buffered_box(nx: int = 12, ny: int = 8, dx: float = 0.01, nbuf: int = 3,
+                 entry_vel: float = 0.05, insert_steps: int = 3, seed: int = 7) -> Case:
+    """A small tank with an inflow buffer zone at the left (zone 1), for
+    parity tests of the buffer pass.  The first fluid column is left empty;
+    the next `nbuf` columns are buffer particles moving at entry_vel; the
+    zone's template is the empty column (hydrostatic pressure, entry_vel), as
+    the reference's buffer1 (internal.f90:105-136).  meta["buffer1"] holds the
+    arguments of Scenario.set_buffer: every insert_steps steps the template
+    column is inserted and buffer particles of the entry cells (the cell
+    column holding the template) at x >= off + 1.5 dx are removed.  The release window of comp_pos (buf_xmin,
+    buf_xmax) lies past the zone."""
+    params = default_params(dx)
+    params.update(dt=2.0e-4, tol=1.0e-12, por=0.4, d50=0.01)
+    off = 3.0 * dx + 0.5 * dx
+    L, H = (nx + 4) * dx, (ny + 6) * dx
+    wx, wy, wt = _walls_box(off, off + L, off, off + H, dx)
+    gx, gy = np.meshgrid(np.arange(1, nx), np.arange(ny), indexing="ij")
+    fx = off + (gx.ravel() + 0.5) * dx
+    fy = off + (gy.ravel() + 0.5) * dx
+    fc = ((gx.ravel() >= nx // 2) & (gy.ravel() < ny // 3)).astype(np.float64)
+    _grid(params, off + L + 4 * dx, off + H + 4 * dx)
+    rv = off + 1.5 * dx
+    params.update(buf_xmin=off + (nbuf + 0.5) * dx, buf_xmax=off + (nx - 1) * dx)
+    c = _assemble(params, "buffered_box", fx, fy, fc, wx, wy, wt, dx, off + ny * dx,
+                  steps=2 * insert_steps, meta=dict(nx=nx, ny=ny, dx=dx, nbuf=nbuf))
+    fl = c.parts["tid"] == TID_FLUID
+    col = np.floor((c.parts["x"] - off) / dx + 1e-9).astype(int)
+    buf = fl & (col >= 1) & (col <= nbuf)
+    c.parts["flags"] = np.where(buf, F_BUFFER, 0).astype(np.uint8)
+    c.parts["u"] = np.where(buf, entry_vel, 0.02 * np.sin(c.parts["y"] * 40.0) * fl)
+    ty = off + (np.arange(ny) + 0.5) * dx
+    tx = np.full(ny, off + 0.5 * dx)
+    tp = RHO * abs(G) * (off + ny * dx - ty)
+    c.meta["buffer1"] = dict(x=tx, y=ty, u=np.full(ny, entry_vel), v=np.zeros(ny), p=tp,
+                             mass=RHO * dx * dx, rho=RHO, vol0=dx * dx, c=-0.5, rv=rv,
+                             interval=insert_steps * params["dt"] * (1.0 - 1e-9))
+    return c

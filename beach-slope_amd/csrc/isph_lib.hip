@@ -19,6 +19,7 @@
 #include "isph_solver.h"
 #include "isph_mg.h"
 #include "isph_batch.h"
+#include "isph_buffer.h"
 
 static_assert(NBR_CAP == ISPH_NBR_CAP, "neighbour capacity mismatch");
 
@@ -30,8 +31,21 @@ struct KTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pool, pending;
 };
 
+// an inflow/outflow buffer zone: device copies of the template and its cell
+// tables, and the insertion clock (main.f90:217-235)
+struct HostZone {
+  bool set = false;
+  int nt = 0;
+  double *tx = nullptr, *ty = nullptr, *tu = nullptr, *tv = nullptr, *tp = nullptr;
+  int32_t *tcell = nullptr, *trank = nullptr, *tcnt = nullptr;
+  uint8_t* entry = nullptr;
+  double mass = 0, rho = 0, vol0 = 0, c = 0, rv = 0, interval = 0, time_shift = 0;
+  int iter = 1;
+};
+
 struct isph_ctx {
   isph_params P;
+  HostZone zones[2];
   bool prof = false;
   KTimer timers[4] = {{"spmv_p"}, {"spmv_s"}, {"spmv_p32"}, {"spmv_s32"}};
   Dev D;
@@ -273,6 +287,10 @@ extern "C" int isph_destroy(isph_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   bh_free(c->self_batch);
+  for (HostZone& z : c->zones) {
+    void* zp[] = {z.tx, z.ty, z.tu, z.tv, z.tp, z.tcell, z.trank, z.tcnt, z.entry};
+    for (void* q : zp) (void)hipFree(q);
+  }
   for (void* p : c->allocs) (void)hipFree(p);
   for (KTimer& t : c->timers) {
     for (auto& pr : t.pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -1179,6 +1197,111 @@ __global__ void k_advance_t(Dev D) {
   if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->t = D.sc->t + D.sc->dt;
 }
 
+// ---------------------------------------------------------------------
+// inflow/outflow buffer zones (internal.f90:447-586, main.f90:217-235)
+// ---------------------------------------------------------------------
+extern "C" int isph_set_buffer(isph_ctx* c, int32_t zone, int64_t nt, const double* x,
+                               const double* y, const double* u, const double* v,
+                               const double* p, double mass, double rho, double vol0,
+                               double cval, double rv, double interval, double time_shift) {
+  if (!c || zone < 1 || zone > 2 || nt < 0 || nt > c->cap) return ISPH_EARG;
+  if (nt > 0 && (!x || !y || !u || !v || !p)) return ISPH_EARG;
+  HostZone& Z = c->zones[zone - 1];
+  std::vector<int32_t> tcell(nt), trank(nt), tcnt(c->ncell, 0);
+  std::vector<uint8_t> entry(c->ncell, 0);
+  for (int64_t t = 0; t < nt; t++) {
+    const int k = locate_cell(c->P, x[t], y[t]);
+    if (k < 0) {
+      set_err(c, "buffer template particle %lld at (%g, %g) is outside the interior cells",
+              (long long)t, x[t], y[t]);
+      return ISPH_EARG;
+    }
+    tcell[t] = k;
+    trank[t] = tcnt[k]++;
+    entry[k] = 1;
+  }
+  void* zp[] = {Z.tx, Z.ty, Z.tu, Z.tv, Z.tp, Z.tcell, Z.trank, Z.tcnt, Z.entry};
+  for (void* q : zp) (void)hipFree(q);
+  Z = HostZone();
+  const size_t m = (size_t)std::max<int64_t>(nt, 1);
+  hipError_t e = hipMalloc((void**)&Z.tx, 8 * m);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.ty, 8 * m);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.tu, 8 * m);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.tv, 8 * m);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.tp, 8 * m);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.tcell, 4 * m);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.trank, 4 * m);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.tcnt, 4 * (size_t)c->ncell);
+  if (e == hipSuccess) e = hipMalloc((void**)&Z.entry, (size_t)c->ncell);
+  if (e != hipSuccess) {
+    set_err(c, "buffer zone allocation: %s", hipGetErrorString(e));
+    return ISPH_EHIP;
+  }
+  if (nt > 0) {
+    HIPCHK(c, hipMemcpy(Z.tx, x, 8 * nt, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(Z.ty, y, 8 * nt, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(Z.tu, u, 8 * nt, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(Z.tv, v, 8 * nt, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(Z.tp, p, 8 * nt, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(Z.tcell, tcell.data(), 4 * nt, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(Z.trank, trank.data(), 4 * nt, hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemcpy(Z.tcnt, tcnt.data(), 4 * (size_t)c->ncell, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(Z.entry, entry.data(), (size_t)c->ncell, hipMemcpyHostToDevice));
+  Z.nt = (int)nt;
+  Z.mass = mass; Z.rho = rho; Z.vol0 = vol0; Z.c = cval; Z.rv = rv;
+  Z.interval = interval; Z.time_shift = time_shift;
+  Z.iter = 1;
+  Z.set = true;
+  return 0;
+}
+
+extern "C" int isph_buffer_update(isph_ctx* c, int32_t zone) {
+  if (!c || zone < 1 || zone > 2) return ISPH_EARG;
+  HostZone& Zh = c->zones[zone - 1];
+  if (!Zh.set) {
+    set_err(c, "buffer zone %d is not set", zone);
+    return ISPH_ESTATE;
+  }
+  if ((int64_t)c->n_hi + Zh.nt > c->cap) {
+    set_err(c, "buffer insertion could exceed capacity %lld", (long long)c->cap);
+    return ISPH_ECAP;
+  }
+  BufZone Z{zone, Zh.nt, Zh.tx, Zh.ty, Zh.tu, Zh.tv, Zh.tp, Zh.tcell, Zh.trank, Zh.tcnt,
+            Zh.entry, Zh.mass, Zh.rho, Zh.vol0, Zh.c, Zh.rv};
+  const int nc = c->ncell;
+  LAUNCH(k_fill_i32, 2 * nc + 1, c->D.bincnt, 2 * nc + 1, 0);
+  LAUNCH(k_buf_rank, nc, c->D, Z, nc);
+  int rc = scan_bins(c, nc + 1);
+  if (rc) return rc;
+  LAUNCH(k_buf_scatter_kept, c->n_hi, c->D);
+  if (Zh.nt > 0) LAUNCH(k_buf_scatter_new, Zh.nt, c->D, Z, nc);
+  swap_state(c->D);
+  const int n_new_hi = c->n_hi + Zh.nt;
+  LAUNCH(k_buf_finish, std::max(nc + 1, n_new_hi), c->D, nc, Zh.nt);
+  hipLaunchKernelGGL(k_buf_set_n, dim3(1), dim3(64), 0, c->stream, c->D, nc);
+  HIPCHK(c, hipGetLastError());
+  set_nhi(c, n_new_hi);
+  c->assembled = false;
+  c->have_nbr = false;
+  return 0;
+}
+
+// the insertion clock of main.f90:217-235 (told = t - dt)
+static int buffer_clock(isph_ctx* c, double told, double t) {
+  for (int z = 0; z < 2; z++) {
+    HostZone& Z = c->zones[z];
+    if (!Z.set || !(Z.interval > 0.0)) continue;
+    if (((told + Z.time_shift) < Z.iter * Z.interval) &&
+        ((t + Z.time_shift) >= Z.iter * Z.interval)) {
+      int rc = isph_buffer_update(c, z + 1);
+      if (rc) return rc;
+      Z.iter = Z.iter + 1;
+    }
+  }
+  return 0;
+}
+
 // one production-loop iteration, main.f90:199-261, in three parts: up to the
 // PPE assembly, the solve, and the rest
 static int step_pre(isph_ctx* c) {
@@ -1195,6 +1318,8 @@ static int step_pre(isph_ctx* c) {
     LAUNCH(k_scalart, c->n_hi, c->D);
     LAUNCH(k_scalarupdate, c->n_hi, c->D, -1.0);
   }
+  rc = buffer_clock(c, s.t, s.t + s.dt);  // the device's t + dt, the same double sum
+  if (rc) return rc;
   LAUNCH(k_projection, c->n_hi, c->D);
   rc = rebin(c, 0);
   if (rc) return rc;

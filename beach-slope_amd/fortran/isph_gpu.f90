@@ -55,6 +55,7 @@ module isph_gpu
   public :: isph_cellshiftalt, isph_freesurf, isph_opt2_shift, isph_densityupdate
   public :: isph_timestep, isph_scalart_anisotropic, isph_scalarupdate
   public :: isph_init_state, isph_step, isph_footprint_bytes
+  public :: isph_set_buffer, isph_buffer_update
   public :: isph_error_message
 
   interface
@@ -192,6 +193,22 @@ module isph_gpu
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: stime
+    end function
+
+    ! inflow/outflow buffer zones (insert_buffer1/2, remove_buffer1/2)
+    integer(c_int) function isph_set_buffer(ctx, zone, nt, x, y, u, v, p, mass, rho, vol0, c, &
+        rv, interval, time_shift) bind(C, name="isph_set_buffer")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: zone
+      integer(c_int64_t), value :: nt
+      real(c_double), intent(in) :: x(*), y(*), u(*), v(*), p(*)
+      real(c_double), value :: mass, rho, vol0, c, rv, interval, time_shift
+    end function
+    integer(c_int) function isph_buffer_update(ctx, zone) bind(C, name="isph_buffer_update")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: zone
     end function
 
     ! fused entries

@@ -168,6 +168,29 @@ int isph_timestep(isph_ctx *ctx);                /* integrator.f90:885-953  */
 int isph_scalart_anisotropic(isph_ctx *ctx);     /* scalar.f90:242-362      */
 int isph_scalarupdate(isph_ctx *ctx, double stime); /* scalar.f90:364-397   */
 
+/* ---- inflow/outflow buffer zones (internal.f90:447-586, main.f90:217-235) ----
+ * isph_set_buffer sets zone 1 or 2: the template particles (x, y, u, v, p,
+ * in the reference's loop order buffer(k,i), k fastest) whose cells are the
+ * zone's entry cells; the values an inserted particle takes (mass, density
+ * -- also its reference density --, volume, concentration c_ref; the
+ * reference's insert_buffer1 uses fmass, rho, fmass/rho, -0.5 and
+ * insert_buffer2 fmass*relden, rho*relden, fmass/rho, +0.5); the removal
+ * coordinate rv (zone 1 drops buffer particles with x >= rv, the reference's
+ * rv_buf_r; zone 2 those with x <= rv, rv_buf_l); and the insertion clock:
+ * isph_step updates the zone after the scalar sub-steps of the step in which
+ * t + time_shift crosses iter * interval (ins_1, ins_2), iter = 1, 2, ...
+ * interval 0 leaves the updates to isph_buffer_update.
+ * isph_buffer_update runs remove_buffer then insert_buffer of the zone: in
+ * each entry cell the kept particles keep their order and the zone's
+ * template particles of the cell follow in template order.  Porosity moves
+ * with its particle (1 for inserted ones); the capacity must hold the
+ * current count plus the template. */
+int isph_set_buffer(isph_ctx *ctx, int32_t zone, int64_t nt, const double *x,
+                    const double *y, const double *u, const double *v,
+                    const double *p, double mass, double rho, double vol0,
+                    double c, double rv, double interval, double time_shift);
+int isph_buffer_update(isph_ctx *ctx, int32_t zone);
+
 /* The two halves of ppesolve, for phase-by-phase parity:
  * assembly (isph.f90:123-456) and solve + pressure scatter (isph.f90:462-486). */
 int isph_ppe_assemble(isph_ctx *ctx);

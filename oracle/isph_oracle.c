@@ -49,8 +49,22 @@
 #define RCAP (ISPH_NBR_CAP + 2)
 #define RED_BLOCK 256
 
+/* an inflow/outflow buffer zone (internal.f90:447-586): the template
+ * particles buffer1/buffer2, their cells (the entry cells), the values an
+ * inserted particle takes, and the insertion clock of main.f90:217-235 */
+typedef struct ozone {
+  int64_t nt;
+  double *tx, *ty, *tu, *tv, *tp;
+  int32_t *tcell;
+  uint8_t *entry;               /* per cell: an entry cell of this zone */
+  double mass, rho, vol0, c, rv, interval, time_shift;
+  int32_t iter;                 /* next insertion is at iter * interval */
+  int set;
+} ozone;
+
 typedef struct ostate {
   isph_params P;
+  ozone zone[2];
   int64_t n, cap, ns, scap;
   int32_t ncell;
   /* particle fields in cell-sorted order (the reference's plist) */
@@ -214,6 +228,11 @@ void or_destroy(ostate *s) {
                   s->nbr, s->nr, s->klt1, s->klt2, s->alen, s->acol, s->aval,
                   s->cstart, s->sx, s->sy, s->sm, s->srho, s->scell, s->sstart};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
+  for (int z = 0; z < 2; z++) {
+    ozone *Z = &s->zone[z];
+    free(Z->tx); free(Z->ty); free(Z->tu); free(Z->tv); free(Z->tp); free(Z->tcell);
+    free(Z->entry);
+  }
   free(s);
 }
 
@@ -1185,15 +1204,149 @@ int or_init_state(ostate *s) {
 }
 
 /* One production-loop iteration, main.f90:199-261. */
+/* ------------------------------------------------------------------ */
+/* inflow/outflow buffers (internal.f90:447-586, main.f90:217-235)        */
+/* ------------------------------------------------------------------ */
+/* Zone 1 or 2.  The template particles (x, y, vx, vy, pressure) come in the
+ * reference's loop order (buffer(k,i), k fastest); their cells are the zone's
+ * entry cells.  An inserted particle takes the template's position, velocity
+ * and pressure and the zone's mass, density (also the reference density),
+ * volume and concentration (insert_buffer1: fmass, rho, fmass/rho, -0.5;
+ * insert_buffer2: fmass*relden, rho*relden, fmass/rho, +0.5).  rv: zone 1
+ * removes buffer particles with x >= rv (remove_buffer1, rv_buf_r), zone 2
+ * those with x <= rv (remove_buffer2, rv_buf_l).  interval (ins_1, ins_2)
+ * and time_shift drive the insertion clock; interval 0 leaves it to
+ * or_buffer_update. */
+int or_set_buffer(ostate *s, int zone, int64_t nt, const double *x, const double *y,
+                  const double *u, const double *v, const double *p, double mass, double rho,
+                  double vol0, double c, double rv, double interval, double time_shift) {
+  if (zone < 1 || zone > 2 || nt < 0) return ISPH_EARG;
+  ozone *Z = &s->zone[zone - 1];
+  free(Z->tx); free(Z->ty); free(Z->tu); free(Z->tv); free(Z->tp); free(Z->tcell);
+  free(Z->entry);
+  size_t m = (size_t)(nt > 0 ? nt : 1);
+  Z->tx = malloc(8 * m); Z->ty = malloc(8 * m); Z->tu = malloc(8 * m);
+  Z->tv = malloc(8 * m); Z->tp = malloc(8 * m); Z->tcell = malloc(4 * m);
+  Z->entry = calloc((size_t)s->ncell, 1);
+  for (int64_t t = 0; t < nt; t++) {
+    int ci, cj;
+    int k = locate(s, x[t], y[t], &ci, &cj);
+    if (k < 0) return ISPH_EARG;
+    Z->tx[t] = x[t]; Z->ty[t] = y[t]; Z->tu[t] = u[t]; Z->tv[t] = v[t]; Z->tp[t] = p[t];
+    Z->tcell[t] = k;
+    Z->entry[k] = 1;
+  }
+  Z->nt = nt;
+  Z->mass = mass; Z->rho = rho; Z->vol0 = vol0; Z->c = c; Z->rv = rv;
+  Z->interval = interval; Z->time_shift = time_shift;
+  Z->iter = 1;
+  Z->set = 1;
+  return 0;
+}
+
+/* remove_buffer then insert_buffer of one zone: in every entry cell the
+ * kept particles stay in order, buffer particles past rv are dropped, and
+ * the zone's template particles of that cell are appended in template
+ * order; other cells are untouched.  Porosity moves with its particle
+ * (inserted particles get 1); the other derived properties are recomputed
+ * before they are read and are poisoned here as after a rebin.  Solution
+ * rows beyond the old particle count start from 0. */
+int or_buffer_update(ostate *s, int zone) {
+  if (zone < 1 || zone > 2 || !s->zone[zone - 1].set) return ISPH_EARG;
+  const ozone *Z = &s->zone[zone - 1];
+  if (s->n + Z->nt > s->cap) return ISPH_ECAP;
+  int64_t *src = malloc(sizeof(int64_t) * (size_t)(s->n + Z->nt + 1));
+  int64_t nn = 0;
+  for (int k = 0; k < s->ncell; k++) {
+    for (int64_t i = s->cstart[k]; i < s->cstart[k + 1]; i++) {
+      int drop = 0;
+      if (Z->entry[k] && (s->flags[i] & ISPH_F_BUFFER))
+        drop = (zone == 1) ? (s->x[i] >= Z->rv) : (s->x[i] <= Z->rv);
+      if (!drop) src[nn++] = i;
+    }
+    if (Z->entry[k])
+      for (int64_t t = 0; t < Z->nt; t++)
+        if (Z->tcell[t] == k) src[nn++] = -1 - t;
+  }
+  /* gather into the scratch vectors, field by field */
+  double *dfield[] = {s->x, s->y, s->u, s->v, s->us, s->vs, s->xs, s->ys, s->p, s->m,
+                      s->rho, s->rho0, s->vol0, s->c, s->por};
+  const int nd = (int)(sizeof(dfield) / sizeof(dfield[0]));
+  double *tmp = malloc(sizeof(double) * (size_t)(nn > 0 ? nn : 1));
+  for (int f = 0; f < nd; f++) {
+    for (int64_t d = 0; d < nn; d++) {
+      const int64_t q = src[d];
+      if (q >= 0) { tmp[d] = dfield[f][q]; continue; }
+      const int64_t t = -1 - q;
+      double v;
+      switch (f) {
+        case 0: v = Z->tx[t]; break;
+        case 1: v = Z->ty[t]; break;
+        case 2: v = Z->tu[t]; break;
+        case 3: v = Z->tv[t]; break;
+        case 8: v = Z->tp[t]; break;
+        case 9: v = Z->mass; break;
+        case 10: case 11: v = Z->rho; break;
+        case 12: v = Z->vol0; break;
+        case 13: v = Z->c; break;
+        case 14: v = 1.0; break;
+        default: v = 0.0; break;   /* us, vs, xs, ys */
+      }
+      tmp[d] = v;
+    }
+    memcpy(dfield[f], tmp, sizeof(double) * (size_t)nn);
+  }
+  int32_t *itmp = malloc(sizeof(int32_t) * (size_t)(nn > 0 ? nn : 1));
+  uint8_t *btmp = malloc((size_t)(nn > 0 ? nn : 1));
+  for (int64_t d = 0; d < nn; d++)
+    itmp[d] = src[d] >= 0 ? s->tid[src[d]] : ISPH_TID_FLUID;
+  memcpy(s->tid, itmp, sizeof(int32_t) * (size_t)nn);
+  for (int64_t d = 0; d < nn; d++)
+    itmp[d] = src[d] >= 0 ? s->cell[src[d]] : Z->tcell[-1 - src[d]];
+  memcpy(s->cell, itmp, sizeof(int32_t) * (size_t)nn);
+  for (int64_t d = 0; d < nn; d++)
+    btmp[d] = src[d] >= 0 ? s->flags[src[d]] : (uint8_t)ISPH_F_BUFFER;
+  memcpy(s->flags, btmp, (size_t)nn);
+  for (int64_t i = s->n; i < nn; i++) s->sol[i] = 0.0;
+  /* keep the porosity gathered above through the poisoning */
+  double *por = tmp;
+  memcpy(por, s->por, sizeof(double) * (size_t)nn);
+  s->n = nn;
+  rebuild_starts(s);
+  poison_props(s);
+  memcpy(s->por, por, sizeof(double) * (size_t)nn);
+  s->assembled = 0;
+  free(src); free(tmp); free(itmp); free(btmp);
+  return 0;
+}
+
+/* the insertion clock of main.f90:217-235, told = t - dt */
+static int buffer_clock(ostate *s, double told) {
+  for (int z = 0; z < 2; z++) {
+    ozone *Z = &s->zone[z];
+    if (!Z->set || !(Z->interval > 0.0)) continue;
+    if (((told + Z->time_shift) < Z->iter * Z->interval) &&
+        ((s->t + Z->time_shift) >= Z->iter * Z->interval)) {
+      int rc = or_buffer_update(s, z + 1);
+      if (rc) return rc;
+      Z->iter = Z->iter + 1;
+    }
+  }
+  return 0;
+}
+
 int or_step(ostate *s, int nsteps, isph_solve_stats *st) {
   int rcs = 0;
   for (int k = 0; k < nsteps; k++) {
     if (s->solsteps < 1 || s->solsteps > ISPH_MAX_SOLSTEPS) return ISPH_ENONFINITE;
+    const double told = s->t;
     s->t = s->t + s->dt;
     for (int q = 0; q < s->solsteps; q++) {
       or_scalart(s);
       or_scalarupdate(s, s->dtsol);
     }
+    int brc = buffer_clock(s, told);
+    if (brc) return brc;
     or_projection(s);
     or_cellshift(s);
     int rc = or_neighbour(s);

@@ -103,6 +103,8 @@ def lib():
         L.or_reduce.argtypes = [i64, vp]
         L.or_threads.restype = C.c_int
         L.or_solve_system.argtypes = [vp, i64, vp, vp, vp, vp, vp, C.POINTER(SolveStats)]
+        L.or_set_buffer.argtypes = [vp, C.c_int, i64] + [vp] * 5 + [d] * 7
+        L.or_buffer_update.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -196,6 +198,15 @@ class Oracle:
         st = SolveStats()
         rc = lib().or_ppe_solve(self.s, C.byref(st))
         return rc, st
+
+    def set_buffer(self, zone: int, x, y, u, v, p, mass: float, rho: float, vol0: float,
+                   c: float, rv: float, interval: float = 0.0, time_shift: float = 0.0) -> int:
+        t = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u, v, p)]
+        return lib().or_set_buffer(self.s, zone, t[0].size, *[_ptr(a) for a in t], mass, rho,
+                                   vol0, c, rv, interval, time_shift)
+
+    def buffer_update(self, zone: int) -> int:
+        return lib().or_buffer_update(self.s, zone)
 
     def step(self, nsteps: int = 1):
         st = SolveStats()
