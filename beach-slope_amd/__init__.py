@@ -74,7 +74,8 @@ class Scalars(C.Structure):
 # entry points taking only the context (one per reference subroutine)
 PHASES = ("projection", "cellshift", "resetid", "neighbour", "effpor", "eddyvis", "int_vel",
           "comp_vel", "comp_pos", "cellshiftalt", "freesurf", "opt2_shift",
-          "densityupdate", "timestep", "scalart_anisotropic", "ppe_assemble", "init_state")
+          "densityupdate", "timestep", "scalart_anisotropic", "ppe_assemble", "init_state",
+          "presstoden", "dentopress", "calmasscons", "updateden")
 
 EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_upload", "isph_upload_solids", "isph_download", "isph_get_scalars",
@@ -82,7 +83,7 @@ EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_step", "isph_dump_cells", "isph_dump_neighbours", "isph_dump_props",
            "isph_dump_ppe", "isph_ppe_set_rhs", "isph_ppe_get_solution",
            "isph_load_system", "isph_batch_create", "isph_batch_step", "isph_batch_stats",
-           "isph_batch_destroy", "isph_set_buffer", "isph_buffer_update",
+           "isph_batch_destroy", "isph_set_buffer", "isph_buffer_update", "isph_set_wcsph",
            "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read") + tuple("isph_" + p for p in PHASES)
 
 _lib = None
@@ -131,6 +132,7 @@ def lib():
     L.isph_batch_destroy.argtypes = [vp]
     L.isph_set_buffer.argtypes = [vp, i32, i64] + [vp] * 5 + [d] * 7
     L.isph_buffer_update.argtypes = [vp, i32]
+    L.isph_set_wcsph.argtypes = [vp, d, d, d, d, i32]
     L.isph_prof_enable.argtypes = [vp, i32]
     L.isph_prof_read.argtypes = [vp, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
     L.isph_footprint_bytes.restype = i64
@@ -316,6 +318,17 @@ class Scenario:
         self._check(lib().isph_set_buffer(self._ctx, zone, t[0].size, *[_ptr(a) for a in t],
                                           mass, rho, vol0, c, rv, interval, time_shift),
                     "isph_set_buffer")
+
+    def set_wcsph(self, c0: float, gamma: float = 7.0, delta: float = 0.1, swl: float = 0.0,
+                  in_step: bool = True):
+        """Weakly compressible form (isph_set_wcsph)."""
+        self._check(lib().isph_set_wcsph(self._ctx, c0, gamma, delta, swl, int(in_step)),
+                    "isph_set_wcsph")
+
+    def presstoden(self): self._phase("presstoden")
+    def dentopress(self): self._phase("dentopress")
+    def calmasscons(self): self._phase("calmasscons")
+    def updateden(self): self._phase("updateden")
 
     def buffer_update(self, zone: int):
         """remove_buffer then insert_buffer of the zone (isph_buffer_update)."""

@@ -34,6 +34,9 @@ struct DevParams {
   double cs;        // incr*h1 = 4h (setup.f90:120-123)
   double k1, k2;    // 10/(28*3.14 h^2), 1/(3*3.14 h^2), real4 products (kernel.f90:61-62)
   double two_h, lim2, wo;
+  // weakly compressible variant: sound speed, Tait exponent, delta-SPH
+  // coefficient, still-water level of the hydrostatic density
+  double c0, gam, delt, swl;
 };
 
 // Device-resident scalars (the module variables t, dt, dtsol, solsteps,
@@ -69,6 +72,7 @@ struct Dev {
   // derived properties
   double *por, *resist, *gradvx, *cf1, *cf2, *cf3, *cf4, *nut, *tke, *varts, *cdiff, *vel;
   double *xnb, *ynb;
+  double* tden;  // continuity update of the weakly compressible variant
   int32_t *ncnt, *nbr;
   int32_t *cstart, *bincnt, *binstart, *pbin, *prank, *scan_tmp;
   // solids

@@ -20,6 +20,7 @@
 #include "isph_mg.h"
 #include "isph_batch.h"
 #include "isph_buffer.h"
+#include "isph_wcsph.h"
 
 static_assert(NBR_CAP == ISPH_NBR_CAP, "neighbour capacity mismatch");
 
@@ -46,6 +47,7 @@ struct HostZone {
 struct isph_ctx {
   isph_params P;
   HostZone zones[2];
+  bool wcsph = false;      // fused step in the weakly compressible form
   bool prof = false;
   KTimer timers[4] = {{"spmv_p"}, {"spmv_s"}, {"spmv_p32"}, {"spmv_s32"}};
   Dev D;
@@ -198,7 +200,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
                    &D.xs2, &D.ys2, &D.p2, &D.m2, &D.rho2, &D.rho02, &D.vol02, &D.c2,
                    &D.por, &D.resist, &D.gradvx, &D.cf1, &D.cf2, &D.cf3, &D.cf4, &D.nut,
                    &D.tke, &D.varts, &D.cdiff, &D.vel, &D.xnb, &D.ynb, &D.b, &D.sol,
-                   &D.r, &D.rh, &D.pp, &D.t1, &D.sv, &D.t2};
+                   &D.r, &D.rh, &D.pp, &D.t1, &D.sv, &D.t2, &D.tden};
   for (auto p : dd) rc |= dalloc(c, p, N);
   uint8_t** bb[] = {&D.tid, &D.flags, &D.vic, &D.tid2, &D.flags2, &D.vic2};
   for (auto p : bb) rc |= dalloc(c, p, N);
@@ -1287,6 +1289,41 @@ extern "C" int isph_buffer_update(isph_ctx* c, int32_t zone) {
   return 0;
 }
 
+// ---------------------------------------------------------------------
+// weakly compressible variant (wcsph.f90:13-154)
+// ---------------------------------------------------------------------
+extern "C" int isph_set_wcsph(isph_ctx* c, double c0, double gamma, double delta,
+                              double swl, int32_t in_step) {
+  if (!c || !(c0 > 0.0) || !(gamma > 0.0)) return ISPH_EARG;
+  c->D.P.c0 = c0;
+  c->D.P.gam = gamma;
+  c->D.P.delt = delta;
+  c->D.P.swl = swl;
+  c->wcsph = in_step != 0;
+  return 0;
+}
+extern "C" int isph_presstoden(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_presstoden, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_dentopress(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_dentopress, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_calmasscons(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  if (!c->have_nbr) { set_err(c, "calmasscons before neighbour"); return ISPH_ESTATE; }
+  LAUNCH(k_calmasscons, c->n_hi, c->D);
+  return 0;
+}
+extern "C" int isph_updateden(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  LAUNCH(k_updateden, c->n_hi, c->D);
+  return 0;
+}
+
 // the insertion clock of main.f90:217-235 (told = t - dt)
 static int buffer_clock(isph_ctx* c, double told, double t) {
   for (int z = 0; z < 2; z++) {
@@ -1329,6 +1366,13 @@ static int step_pre(isph_ctx* c) {
   LAUNCH(k_eddyvis, c->n_hi, c->D);
   LAUNCH(k_int_vel_varts, c->n_hi, c->D);
   LAUNCH(k_int_vel, c->n_hi, c->D);
+  if (c->wcsph) {
+    // continuity and the Tait law in place of the pressure Poisson solve
+    LAUNCH(k_calmasscons, c->n_hi, c->D);
+    LAUNCH(k_updateden, c->n_hi, c->D);
+    LAUNCH(k_dentopress, c->n_hi, c->D);
+    return check_err_flags(c);
+  }
   rc = assemble_nocheck(c);
   if (rc) return rc;
   return check_err_flags(c);
@@ -1345,7 +1389,7 @@ static int step_post(isph_ctx* c) {
   LAUNCH(k_effpor, c->n_hi, c->D);
   LAUNCH(k_opt2_shift, c->n_hi, c->D);
   LAUNCH(k_shift_apply, c->n_hi, c->D);
-  LAUNCH(k_densityupdate, c->n_hi, c->D);
+  if (!c->wcsph) LAUNCH(k_densityupdate, c->n_hi, c->D);
   rc = isph_timestep(c);
   if (rc) return rc;
   LAUNCH(k_eddyvis, c->n_hi, c->D);
@@ -1360,6 +1404,10 @@ static bool solve_fatal(int rc) {
 static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
   int rc = step_pre(c);
   if (rc) return rc;
+  if (c->wcsph) {
+    if (stats) *stats = isph_solve_stats{};
+    return step_post(c);
+  }
   rc = ppe_solve_impl(c, stats);
   if (solve_fatal(rc)) return rc;
   *solve_rc = rc;
@@ -1518,7 +1566,8 @@ extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, cons
 // batch is switched to one shared stream; the phases before and after the
 // PPE solve are launched per scenario, and the precision-form solve of all
 // scenarios runs as one set of launches (bh_solve).  Contexts in the FP64
-// form are solved one after the other.
+// form are solved one after the other; weakly compressible contexts (set
+// before the batch is made) have no solve.
 struct isph_batch {
   std::vector<isph_ctx*> ctx;
   std::vector<hipStream_t> own;  // each context's stream, restored on destroy
@@ -1550,7 +1599,7 @@ extern "C" int isph_batch_create(int32_t n_scen, isph_ctx* const* ctxs, isph_bat
     (void)hipStreamSynchronize(c->stream);
     b->own.push_back(c->stream);
     c->stream = b->stream;
-    if (c->P.precision != ISPH_PPE_FP64) b->pctx.push_back(c);
+    if (c->P.precision != ISPH_PPE_FP64 && !c->wcsph) b->pctx.push_back(c);
   }
   if (!b->pctx.empty()) {
     int rc = bh_make(b->pctx.data(), (int)b->pctx.size(), &b->pb);
@@ -1582,7 +1631,9 @@ extern "C" int isph_batch_step(isph_batch* b, int32_t nsteps) {
       isph_ctx* c = b->ctx[s];
       isph_solve_stats& st = b->last[s];
       int rc = 0;
-      if (c->P.precision != ISPH_PPE_FP64) {
+      if (c->wcsph) {
+        st = isph_solve_stats{};  // no pressure solve in the weakly compressible form
+      } else if (c->P.precision != ISPH_PPE_FP64) {
         st = pst[q++];
         c->last = st;
         if (!std::isfinite(st.rel_residual) || !std::isfinite(st.bnorm)) {

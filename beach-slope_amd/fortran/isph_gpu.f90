@@ -56,6 +56,7 @@ module isph_gpu
   public :: isph_timestep, isph_scalart_anisotropic, isph_scalarupdate
   public :: isph_init_state, isph_step, isph_footprint_bytes
   public :: isph_set_buffer, isph_buffer_update
+  public :: isph_set_wcsph, isph_presstoden, isph_dentopress, isph_calmasscons, isph_updateden
   public :: isph_error_message
 
   interface
@@ -209,6 +210,31 @@ module isph_gpu
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: zone
+    end function
+
+    ! weakly compressible variant (wcsph.f90)
+    integer(c_int) function isph_set_wcsph(ctx, co, t_gam, delt, swl, in_step) &
+        bind(C, name="isph_set_wcsph")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: co, t_gam, delt, swl
+      integer(c_int32_t), value :: in_step
+    end function
+    integer(c_int) function isph_presstoden(ctx) bind(C, name="isph_presstoden")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function isph_dentopress(ctx) bind(C, name="isph_dentopress")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function isph_calmasscons(ctx) bind(C, name="isph_calmasscons")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function isph_updateden(ctx) bind(C, name="isph_updateden")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
 
     ! fused entries

@@ -191,6 +191,27 @@ int isph_set_buffer(isph_ctx *ctx, int32_t zone, int64_t nt, const double *x,
                     double c, double rv, double interval, double time_shift);
 int isph_buffer_update(isph_ctx *ctx, int32_t zone);
 
+/* ---- weakly compressible variant (wcsph.f90:13-154) ----
+ * Dead and not compiling in the reference; restated from its equations.
+ * isph_set_wcsph sets the sound speed co, the Tait exponent t_gam (7 in the
+ * reference), the delta-SPH coefficient delt (0.1) and the still-water level
+ * swl of the hydrostatic density (brrealy*distfac*(2*bl-1) + wc); with
+ * in_step != 0, isph_step replaces the PPE assembly and solve by
+ * calmasscons, updateden, dentopress and skips densityupdate (the density
+ * is a state variable).  One entry per reference subroutine:
+ *   presstoden   density from pressure, rho ((p gam/(rho co^2)) + 1)^(1/gam)
+ *   dentopress   pressure from density (fluid), clipped at 0
+ *   calmasscons  continuity + delta-SPH term into tden (fluid); the
+ *                reference's y-component sum, overwritten at every
+ *                neighbour by a variable reuse, has its own accumulator
+ *   updateden    density = tden (fluid) */
+int isph_set_wcsph(isph_ctx *ctx, double co, double t_gam, double delt, double swl,
+                   int32_t in_step);
+int isph_presstoden(isph_ctx *ctx);
+int isph_dentopress(isph_ctx *ctx);
+int isph_calmasscons(isph_ctx *ctx);
+int isph_updateden(isph_ctx *ctx);
+
 /* The two halves of ppesolve, for phase-by-phase parity:
  * assembly (isph.f90:123-456) and solve + pressure scatter (isph.f90:462-486). */
 int isph_ppe_assemble(isph_ctx *ctx);

@@ -65,6 +65,12 @@ typedef struct ozone {
 typedef struct ostate {
   isph_params P;
   ozone zone[2];
+  /* WCSPH (wcsph.f90): sound speed co, exponent t_gam, delta-SPH coefficient
+   * delt, still-water level of the hydrostatic density; in_step switches
+   * the fused step to the weakly compressible form */
+  double c0, gam, delt, swl;
+  int wcsph;
+  double *tden;
   int64_t n, cap, ns, scap;
   int32_t ncell;
   /* particle fields in cell-sorted order (the reference's plist) */
@@ -190,7 +196,8 @@ ostate *or_create(const isph_params *P, int64_t cap, int64_t scap) {
   double **dp[] = {&s->x, &s->y, &s->u, &s->v, &s->us, &s->vs, &s->xs, &s->ys, &s->p, &s->m,
                    &s->rho, &s->rho0, &s->vol0, &s->c, &s->por, &s->resist, &s->gradvx,
                    &s->nut, &s->tke, &s->varts, &s->cdiff, &s->vel, &s->b, &s->sol,
-                   &s->w1, &s->w2, &s->w3, &s->w4, &s->w5, &s->w6, &s->w7, &s->w8};
+                   &s->w1, &s->w2, &s->w3, &s->w4, &s->w5, &s->w6, &s->w7, &s->w8,
+                   &s->tden};
   for (size_t k = 0; k < sizeof(dp) / sizeof(dp[0]); k++) ALLOC(*dp[k], cap);
   ALLOC(s->coff, 4 * cap);
   ALLOC(s->tid, cap);
@@ -226,7 +233,7 @@ void or_destroy(ostate *s) {
                   s->varts, s->cdiff, s->vel, s->b, s->sol, s->w1, s->w2, s->w3, s->w4,
                   s->w5, s->w6, s->w7, s->w8, s->coff, s->tid, s->cell, s->flags, s->perm, s->cls, s->ncnt,
                   s->nbr, s->nr, s->klt1, s->klt2, s->alen, s->acol, s->aval,
-                  s->cstart, s->sx, s->sy, s->sm, s->srho, s->scell, s->sstart};
+                  s->cstart, s->sx, s->sy, s->sm, s->srho, s->scell, s->sstart, s->tden};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
   for (int z = 0; z < 2; z++) {
     ozone *Z = &s->zone[z];
@@ -1320,6 +1327,91 @@ int or_buffer_update(ostate *s, int zone) {
   return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* WCSPH variant (wcsph.f90:13-154; dead and not compiling in the       */
+/* reference, restated from its equations)                               */
+/* ------------------------------------------------------------------ */
+void or_set_wcsph(ostate *s, double c0, double gam, double delt, double swl, int in_step) {
+  s->c0 = c0; s->gam = gam; s->delt = delt; s->swl = swl; s->wcsph = in_step;
+}
+
+/* density of the hydrostatic pressure at height y (the term2 expression of
+ * calmasscons, wcsph.f90:87-91, with swl = brrealy*distfac*(2*bl-1) + wc) */
+static double rho_hyd(const ostate *s, double y) {
+  const double rho = s->P.rho;
+  return rho * pow(((((s->swl - y) * rho) * fabs(s->P.g)) * s->gam) / (rho * (s->c0 * s->c0)) + 1.0,
+                   1.0 / s->gam);
+}
+
+/* presstoden, wcsph.f90:13-33: density from pressure, every particle */
+void or_presstoden(ostate *s) {
+  const double rho = s->P.rho;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < s->n; i++)
+    s->rho[i] = rho * pow(((s->p[i] * s->gam) / (rho * (s->c0 * s->c0))) + 1.0, 1.0 / s->gam);
+}
+
+/* dentopress, wcsph.f90:35-63: Tait pressure of the fluid, clipped at 0
+ * (then the density is reset to rho) */
+void or_dentopress(ostate *s) {
+  const double rho = s->P.rho;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < s->n; i++) {
+    if (s->tid[i] <= 2) continue;
+    s->p[i] = ((rho * (s->c0 * s->c0)) * (pow(s->rho[i] / rho, s->gam) - 1.0)) / s->gam;
+    if (s->p[i] < 0.0) {
+      s->p[i] = 0.0;
+      s->rho[i] = rho;
+    }
+  }
+}
+
+/* calmasscons, wcsph.f90:65-130: continuity with the delta-SPH diffusion
+ * term taken relative to the hydrostatic density,
+ *   drho_i/dt = sum_j [ -m_j (u_j - u_i) . gradW~ (rho_i / rho_j)
+ *               + delt 2h c0 (rho_j - rho_i - (rhoH_j - rhoH_i)) (x_j - x_i) . gradW~
+ *                 (m_j / rho_j) / (r^2 + lam) ],
+ * per component in the reference's expression order, gradW~ the corrected
+ * gradient with respect to x_i (klt = Wabx(neighbour, particle),
+ * search.f90:70-73).  Two departures from the routine as written (it is never
+ * called there): it keeps the y-component sum in the variable that also holds
+ * the hydrostatic difference, so its y-sum is overwritten at every neighbour
+ * (here the y-sum has its own accumulator); and it takes (x_i - x_j), which
+ * makes the delta term anti-diffusive, where the delta-SPH scheme
+ * (Molteni & Colagrossi 2009) takes (x_j - x_i). */
+void or_calmasscons(ostate *s) {
+  const double h = s->P.h, lam = s->P.lam, c0 = s->c0, delt = s->delt, dt = s->dt;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < s->n; i++) {
+    if (s->tid[i] <= 2) continue;
+    const double *cf = &s->coff[4 * i];
+    const double rhi = rho_hyd(s, s->y[i]);
+    double tx = 0.0, ty = 0.0;
+    for (int mm = 0; mm < s->ncnt[i]; mm++) {
+      const int32_t j = s->nbr[i * KCAP + mm];
+      const double r = s->nr[i * KCAP + mm];
+      const double z1 = s->klt1[i * KCAP + mm], z2 = s->klt2[i * KCAP + mm];
+      const double gx = cf[0] * z1 + cf[1] * z2, gy = cf[2] * z1 + cf[3] * z2;
+      const double hyd = rho_hyd(s, s->y[j]) - rhi;
+      const double vj = s->m[j] / s->rho[j];
+      const double ratio = s->rho[i] / s->rho[j];
+      const double dif = (s->rho[j] - s->rho[i]) - hyd;
+      const double den = r * r + lam;
+      tx = (tx - ((s->m[j] * (s->u[j] - s->u[i])) * gx) * ratio) +
+           ((((((delt * 2) * h) * c0) * dif) * (s->x[j] - s->x[i])) * gx) * vj / den;
+      ty = (ty - ((s->m[j] * (s->v[j] - s->v[i])) * gy) * ratio) +
+           ((((((delt * 2) * h) * c0) * dif) * (s->y[j] - s->y[i])) * gy) * vj / den;
+    }
+    s->tden[i] = s->rho[i] + (tx + ty) * dt;
+  }
+}
+
+/* updateden, wcsph.f90:132-152 */
+void or_updateden(ostate *s) {
+  for (int64_t i = 0; i < s->n; i++)
+    if (s->tid[i] > 2) s->rho[i] = s->tden[i];
+}
+
 /* the insertion clock of main.f90:217-235, told = t - dt */
 static int buffer_clock(ostate *s, double told) {
   for (int z = 0; z < 2; z++) {
@@ -1354,11 +1446,20 @@ int or_step(ostate *s, int nsteps, isph_solve_stats *st) {
     or_effpor(s);
     or_eddyvis(s);
     or_int_vel(s);
-    rc = or_ppe_assemble(s);
-    if (rc) return rc;
-    rc = or_ppe_solve(s, st);
-    if (rc == ISPH_ENONFINITE) return rc;
-    if (rc) rcs = rc;
+    if (s->wcsph) {
+      /* weakly compressible form: continuity and the Tait law in place of
+       * the pressure Poisson solve */
+      or_calmasscons(s);
+      or_updateden(s);
+      or_dentopress(s);
+      if (st) memset(st, 0, sizeof *st);
+    } else {
+      rc = or_ppe_assemble(s);
+      if (rc) return rc;
+      rc = or_ppe_solve(s, st);
+      if (rc == ISPH_ENONFINITE) return rc;
+      if (rc) rcs = rc;
+    }
     or_comp_vel(s);
     or_comp_pos(s);
     or_cellshiftalt(s);
@@ -1367,7 +1468,7 @@ int or_step(ostate *s, int nsteps, isph_solve_stats *st) {
     or_freesurf(s);
     or_effpor(s);
     or_opt2_shift(s);
-    or_densityupdate(s);
+    if (!s->wcsph) or_densityupdate(s);
     or_timestep(s);
     or_eddyvis(s);
   }
@@ -1387,7 +1488,7 @@ void *or_array(ostate *s, const char *name) {
       {"cdiff", s->cdiff}, {"vel", s->vel}, {"ncnt", s->ncnt}, {"nbr", s->nbr}, {"nr", s->nr},
       {"klt1", s->klt1}, {"klt2", s->klt2}, {"cstart", s->cstart}, {"alen", s->alen},
       {"acol", s->acol}, {"aval", s->aval}, {"b", s->b}, {"sol", s->sol}, {"sx", s->sx},
-      {"sy", s->sy}, {"sm", s->sm}, {"srho", s->srho}, {"sstart", s->sstart}};
+      {"sy", s->sy}, {"sm", s->sm}, {"srho", s->srho}, {"sstart", s->sstart}, {"tden", s->tden}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); k++)
     if (strcmp(t[k].n, name) == 0) return t[k].p;
   return NULL;

@@ -79,7 +79,8 @@ def lib():
         L.or_array.argtypes = [vp, C.c_char_p]
         for f in ("or_projection", "or_cellshift", "or_cellshiftalt", "or_effpor",
                   "or_eddyvis", "or_int_vel", "or_comp_vel", "or_comp_pos", "or_freesurf",
-                  "or_opt2_shift", "or_densityupdate", "or_timestep", "or_scalart"):
+                  "or_opt2_shift", "or_densityupdate", "or_timestep", "or_scalart",
+                  "or_presstoden", "or_dentopress", "or_calmasscons", "or_updateden"):
             getattr(L, f).argtypes = [vp]
             getattr(L, f).restype = None
         for f in ("or_neighbour", "or_ppe_assemble", "or_init_state"):
@@ -105,6 +106,7 @@ def lib():
         L.or_solve_system.argtypes = [vp, i64, vp, vp, vp, vp, vp, C.POINTER(SolveStats)]
         L.or_set_buffer.argtypes = [vp, C.c_int, i64] + [vp] * 5 + [d] * 7
         L.or_buffer_update.argtypes = [vp, C.c_int]
+        L.or_set_wcsph.argtypes = [vp, d, d, d, d, C.c_int]
         _lib = L
     return _lib
 
@@ -171,6 +173,16 @@ class Oracle:
         buf = (C.c_char * (cnt * w * np.dtype(dt).itemsize)).from_address(ptr)
         a = np.frombuffer(buf, dtype=dt).copy()
         return a.reshape(cnt, w) if w > 1 else a
+
+    def put(self, name: str, values) -> None:
+        """Overwrite an internal per-particle array (live particles) in place."""
+        dt, w = _DTYPES.get(name, (np.float64, 1))
+        ptr = lib().or_array(self.s, name.encode())
+        if not ptr:
+            raise KeyError(name)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=dt),
+                                                 (self.n, w) if w > 1 else (self.n,)))
+        C.memmove(ptr, a.ctypes.data, a.nbytes)
 
     def cstart(self) -> np.ndarray:
         ncell = self.P.cellx * self.P.celly
