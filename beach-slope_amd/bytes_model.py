@@ -4,8 +4,9 @@ Each kernel is charged for reading every input array once and writing every
 output once; values gathered at neighbour rows are assumed to come from cache
 (they are rows of the same vectors).  n = rows, nnz = stored PPE entries.
 Per row: the row length (int32), the row's own vector entries read, the
-outputs written; per entry: the column index (int32) and the value (FP64 in
-the plain form, FP32 in the precision form).
+outputs written; per entry: the column (the int32 index and an FP64 value in
+the plain form; the 16-bit offset j - i and an FP32 value in the precision
+form).
 
   k_bicg_1  t1 = A p, partial (t1 . rh)       reads alen, p, rh; writes t1
   k_bicg_2  t2 = A s, s = r - alpha t1 on the fly, partials (t2 . s), (t2 . t2)
@@ -21,7 +22,7 @@ _PER_ROW = {
     "kb_pb_1": (4 + 4 + 4, 4),
     "kb_pb_2": (4 + 4 + 8, 4),
 }
-_PER_NNZ = {"k_bicg_1": 12, "k_bicg_2": 12, "kb_pb_1": 8, "kb_pb_2": 8}
+_PER_NNZ = {"k_bicg_1": 12, "k_bicg_2": 12, "kb_pb_1": 6, "kb_pb_2": 6}
 
 
 def kernel_bytes(kernel: str, n: int, nnz: float) -> float:

@@ -19,7 +19,7 @@
 enum {
   BK_ROWS = 0,                           // particle rows
   BK_CELL1 = 1,                          // level-1 cells
-  BK_DOWN = 2,                           // + l-1: level-(l+1) aggregates, l <= nmb
+  BK_DOWN = 2,                           // + l-1: 4 x level-(l+1) aggregates, l <= nmb
   BK_UP = 2 + MG_MAX_LEVELS,             // + l-1: level-l cells, l <= nmb
   BK_COARSEN = 2 + 2 * MG_MAX_LEVELS,    // + l-1: level-(l+1) cells, l < nlev
   BK_NKIND = 2 + 3 * MG_MAX_LEVELS
@@ -82,14 +82,15 @@ __global__ void __launch_bounds__(256) kb_mg_dense(BatchDev B) {
   mg_dense_block(B.mgs[blockIdx.x]);
 }
 
-// ---- V-cycle: WHICH 0 applies M to p (pp32 -> r32), 1 to s (sv -> sv32) ----
-template <int WHICH>
+// ---- V-cycle: V & 1 = 0 applies M to p (pp32 -> r32), 1 to s (sv -> sv32);
+// V & 2: the rows' packed 16-bit column offsets instead of the int32 columns
+template <int V>
 __global__ void __launch_bounds__(256) kb_mg_pre0(BatchDev B) {
   BATCH_ROW(B);
   const MgDev& G = B.mgs[s_];
   const int i = vb_ * RED_BLOCK + threadIdx.x;
-  if (WHICH == 0) mg_pre0_row<float>(D, G, D.pp32, D.d32, i);
-  else mg_pre0_row<double>(D, G, D.sv, D.d32, i);
+  if ((V & 1) == 0) mg_pre0_row<(V & 2) != 0, float>(D, G, D.pp32, D.d32, i);
+  else mg_pre0_row<(V & 2) != 0, double>(D, G, D.sv, D.d32, i);
 }
 __global__ void __launch_bounds__(256) kb_mg_restrict0(BatchDev B) {
   BATCH_CELLS(B, BK_CELL1);
@@ -100,35 +101,42 @@ __global__ void __launch_bounds__(256) kb_mg_down(BatchDev B, int l) {
   mg_lvl_down(G, l, I_);
 }
 __global__ void __launch_bounds__(MG_CO_THREADS) kb_mg_coarse(BatchDev B) {
-  mg_coarse_block(B.mgs[blockIdx.x]);
+  extern __shared__ double mg_lds[];
+  mg_coarse_block(B.mgs[blockIdx.x], mg_lds);
 }
 __global__ void __launch_bounds__(256) kb_mg_up(BatchDev B, int l) {
   BATCH_CELLS(B, BK_UP + l - 1);
   mg_lvl_up(G, l, I_);
 }
-template <int WHICH>
+__global__ void __launch_bounds__(256) kb_mg_prolong0(BatchDev B) {
+  BATCH_ROW(B);
+  mg_prolong0_row(D, B.mgs[s_], vb_ * RED_BLOCK + threadIdx.x);
+}
+template <int V>
 __global__ void __launch_bounds__(256) kb_mg_post0(BatchDev B) {
   BATCH_ROW(B);
   const MgDev& G = B.mgs[s_];
   const int i = vb_ * RED_BLOCK + threadIdx.x;
-  if (WHICH == 0) mg_post0_row<float>(D, G, D.pp32, D.r32, i);
-  else mg_post0_row<double>(D, G, D.sv, D.sv32, i);
+  if ((V & 1) == 0) mg_post0_row<(V & 2) != 0, float>(D, G, D.pp32, D.r32, i);
+  else mg_post0_row<(V & 2) != 0, double>(D, G, D.sv, D.sv32, i);
 }
 
 // ---- BiCGStab rows ----
+template <bool OFF>
 __global__ void __launch_bounds__(256) kb_pb_1(BatchDev B) {
   __shared__ double lds[RED_BLOCK];
   BATCH_ROW(B);
-  pb_1_rows(D, vb_, lds);
+  pb_1_rows<OFF>(D, vb_, lds);
 }
 __global__ void __launch_bounds__(256) kb_pb_s(BatchDev B) {
   BATCH_ROW(B);
-  pb_s_rows(D, vb_);
+  pb_s_rows(D, B.mgs[s_], vb_);
 }
+template <bool OFF>
 __global__ void __launch_bounds__(256) kb_pb_2(BatchDev B) {
   __shared__ double lds[RED_BLOCK];
   BATCH_ROW(B);
-  pb_2_rows(D, vb_, lds);
+  pb_2_rows<OFF>(D, vb_, lds);
 }
 __global__ void __launch_bounds__(256) kb_pb_3(BatchDev B) {
   __shared__ double lds[RED_BLOCK];
@@ -137,12 +145,12 @@ __global__ void __launch_bounds__(256) kb_pb_3(BatchDev B) {
 }
 __global__ void __launch_bounds__(256) kb_pb_4(BatchDev B) {
   BATCH_ROW(B);
-  pb_4_rows(D, vb_);
+  pb_4_rows(D, B.mgs[s_], vb_);
 }
 __global__ void __launch_bounds__(256) kb_pb_resid(BatchDev B, int init) {
   __shared__ double lds[RED_BLOCK];
   BATCH_ROW(B);
-  pb_resid_rows(D, vb_, init, lds);
+  pb_resid_rows(D, B.mgs[s_], vb_, init, lds);
 }
 __global__ void __launch_bounds__(256) kb_pb_bnorm(BatchDev B) {
   __shared__ double lds[RED_BLOCK];
@@ -154,6 +162,7 @@ __global__ void __launch_bounds__(256) kb_copy_sol_to_p(BatchDev B) {
   BATCH_ROW(B);
   const int i = vb_ * RED_BLOCK + threadIdx.x;
   if (i < D.sc->n) D.p[i] = D.sol[i];
+  if (vb_ == 0 && threadIdx.x == 0) D.sc->far = 0;  // set again by the next setup if needed
 }
 
 // one workgroup per scenario

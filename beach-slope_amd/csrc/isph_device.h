@@ -49,6 +49,7 @@ struct DevScalars {
   double rho, alpha, omega, beta, rr, bnorm, stop, rel;
   int iters, done, conv, brk;
   int fin, restarts;  // precision form: scenario finished; restarts from the FP64 residual
+  int far;            // precision form: a column offset of the rows exceeds 16 bits
   unsigned int counter[4];
   unsigned long long nnz;  // stored PPE entries of the last assembly
 };
@@ -82,6 +83,8 @@ struct Dev {
   int32_t *alen, *acol;
   double *aval, *b, *sol;
   float* aval32;
+  uint32_t* aoff;   // precision form: column offsets j - i as 16-bit pairs,
+                    // slots 2w and 2w+1 of row i in word [w * cap + i]
   double *r, *rh, *pp, *t1, *sv, *t2, *part;
   float *r32, *rh32, *pp32, *t132, *sv32, *t232, *d32;
 };
@@ -130,6 +133,46 @@ __device__ __forceinline__ A ell_row_sum(const int32_t* __restrict__ col,
   return acc;
 }
 #define ELL_U 8
+
+// The same walk over 16-bit column offsets j - i packed two to a word (a
+// quarter of the index bytes and half the index loads of the int32
+// columns); used when every offset of the rows fits 16 bits.  U is even.
+template <int U, typename A, typename VAL, typename G>
+__device__ __forceinline__ A ell_row_sum_off(const uint32_t* __restrict__ off,
+                                             const VAL* __restrict__ val, int cap, int i, int L,
+                                             G gather) {
+  A acc = (A)0;
+  for (int k0 = 0; k0 < L; k0 += U) {
+    int c[U];
+    VAL v[U];
+#pragma unroll
+    for (int q = 0; q < U / 2; q++) {
+      const uint32_t wd = (k0 + 2 * q < L) ? off[((k0 >> 1) + q) * cap + i] : 0u;
+      c[2 * q] = i + (int)(int16_t)(wd & 0xffffu);
+      c[2 * q + 1] = i + (int)(int16_t)(wd >> 16);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = (k0 + u < L) ? val[(k0 + u) * cap + i] : (VAL)0;
+    A g[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) g[u] = (k0 + u < L) ? gather(c[u]) : (A)0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u < L) acc = acc + v[u] * g[u];
+  }
+  return acc;
+}
+
+// row sum over int32 columns or packed offsets
+#ifndef ELL_UP
+#define ELL_UP 8   // entries per batch of the precision form's FP32 row walks
+#endif
+template <bool OFF, int U, typename A, typename VAL, typename G>
+__device__ __forceinline__ A ell_row_sum_any(const Dev& D, const VAL* __restrict__ val, int i,
+                                             G gather) {
+  if (OFF) return ell_row_sum_off<U, A>(D.aoff, val, D.cap, i, D.alen[i], gather);
+  return ell_row_sum<U, A>(D.acol, val, D.cap, i, D.alen[i], gather);
+}
 
 // ---------------------------------------------------------------------
 // kernel.f90 restatement (device)

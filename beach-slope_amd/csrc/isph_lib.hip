@@ -63,6 +63,10 @@ struct isph_ctx {
   double* d_bn2 = nullptr;
   MgDev G{};               // multigrid hierarchy (precision form)
   struct BatchHost* self_batch = nullptr;  // this context as a batch of one (precision solve)
+  int co_lds = 0;                          // LDS bytes of its coarse multigrid workgroup
+  // previous precision solve: iterations to the first stop, and after it
+  // (chunk sizing only; the results do not depend on the chunks)
+  int pred_first = 0, pred_after = 0;
 };
 
 #define HIPCHK(ctx, expr)                                                     \
@@ -157,8 +161,9 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   per += 4 * 3;                                 // bin, rank, scan scratch
   per += 4 + ROW_CAP * (4 + 8) + 2 * 8;         // PPE rows, rhs, solution
   per += 6 * 8;                                 // FP64 solver vectors
-  if (precision) per += ROW_CAP * 4 + 7 * 4 + 2 * 4 + 9 * 8;  // FP32 matrix and vectors,
-                                                             // level-0 multigrid vectors, setup rows
+  // precision form: FP32 matrix, packed column offsets, FP32 vectors, level-0
+  // multigrid vectors, setup rows
+  if (precision) per += ROW_CAP * 4 + 4 * ((ROW_CAP + 1) / 2) + 7 * 4 + 2 * 4 + 9 * 8;
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
   b += 8 * 2 * (n_cap / 256 + 2);               // reduction partials
   if (precision) b += 4 * 12 * ncell + 8 * MG_DENSE_MAX * MG_DENSE_MAX;  // cell levels
@@ -228,6 +233,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   rc |= dalloc(c, &c->d_bn2, 1);
   if (params->precision) {
     rc |= dalloc(c, &D.aval32, N * ROW_CAP);
+    rc |= dalloc(c, &D.aoff, N * ((ROW_CAP + 1) / 2));
     float** ff[] = {&D.r32, &D.rh32, &D.pp32, &D.t132, &D.sv32, &D.t232, &D.d32};
     for (auto p : ff) rc |= dalloc(c, p, N);
     // cell hierarchy: level 1 is the cell grid, then 2x2 aggregation down to
@@ -260,11 +266,30 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   }
   c->G.done = &D.sc->done;
   if (params->precision) {
-    // multi-block cell levels while the next level exceeds MG_MB_MIN unknowns
+    // The coarse workgroup runs the longest tail of levels l0..nlev whose
+    // first level has at most MG_CO_PER unknowns per thread and whose data
+    // fits the LDS budget; levels 1..l0-1 run multi-workgroup.
     MgDev& G = c->G;
-    G.nmb = 0;
-    while (G.nmb + 1 < G.nlev && G.lev[G.nmb + 2].n > MG_MB_MIN) G.nmb++;
-    if (G.nlev >= 2 && G.nmb == 0) G.nmb = 1;
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+        hipSuccess)
+      lds_max = 64 * 1024;
+    int64_t budget = std::min<int64_t>(lds_max, MG_CO_LDS_MAX) - 1024;
+    // dynamic LDS above 64 KB is requested explicitly; if the runtime
+    // refuses, the coarse tail is sized for 64 KB
+    if (budget > 63 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&kb_mg_coarse),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)budget) != hipSuccess) {
+      (void)hipGetLastError();
+      budget = 63 * 1024;
+    }
+    int l0 = 1;
+    while (l0 < G.nlev && (G.lev[l0].n > MG_CO_PER * MG_CO_THREADS ||
+                           mg_coarse_lds_bytes(G, l0) > budget))
+      l0++;
+    G.nmb = l0 - 1;
+    c->co_lds = (int)mg_coarse_lds_bytes(G, l0);
     // level 1's final correction: rb when level 1 runs multi-block, else xb
     G.e1 = (G.nmb >= 1) ? G.rb + G.lev[1].off : G.xb + G.lev[1].off;
   }
@@ -771,6 +796,7 @@ struct BatchHost {
   std::vector<MgDev> mgs;
   std::vector<DevScalars> sc;
   int max_nmb = 0, max_nlev = 0;
+  int co_lds = 0;                // LDS bytes of the coarse launch (largest scenario's)
 };
 
 static void bh_free(BatchHost* b) {
@@ -806,6 +832,7 @@ static int bh_make(isph_ctx* const* ctxs, int S, BatchHost** out) {
     const MgDev& G = ctxs[s]->G;
     b->max_nmb = std::max(b->max_nmb, G.nmb);
     b->max_nlev = std::max(b->max_nlev, G.nlev);
+    b->co_lds = std::max(b->co_lds, ctxs[s]->co_lds);
   }
   for (int s = 0; s < S; s++) {
     const MgDev& G = ctxs[s]->G;
@@ -814,7 +841,7 @@ static int bh_make(isph_ctx* const* ctxs, int S, BatchHost** out) {
     };
     put(BK_CELL1, grid_of(G.lev[1].n));
     for (int l = 1; l < MG_MAX_LEVELS; l++) {
-      put(BK_DOWN + l - 1, (l <= G.nmb) ? grid_of(G.lev[l + 1].n) : 0);
+      put(BK_DOWN + l - 1, (l <= G.nmb) ? grid_of(4 * G.lev[l + 1].n) : 0);
       put(BK_UP + l - 1, (l <= G.nmb) ? grid_of(G.lev[l].n) : 0);
       put(BK_COARSEN + l - 1, (l < G.nlev) ? grid_of(G.lev[l + 1].n) : 0);
     }
@@ -854,9 +881,10 @@ static int bh_gather(BatchHost* b) {
   return 0;
 }
 
-#define BLAUNCH(kern, grid, block, ...)                                               \
+#define BLAUNCH(kern, grid, block, ...) BLAUNCH_LDS(kern, grid, block, 0, __VA_ARGS__)
+#define BLAUNCH_LDS(kern, grid, block, lds, ...)                                      \
   do {                                                                                \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, __VA_ARGS__);     \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, c->stream, __VA_ARGS__);   \
     hipError_t e_ = hipGetLastError();                                                \
     if (e_ != hipSuccess) {                                                           \
       set_err(c, "launch of %s failed: %s", #kern, hipGetErrorString(e_));           \
@@ -882,20 +910,46 @@ static int bh_gather(BatchHost* b) {
     }                                                                                 \
   } while (0)
 
-// one V-cycle for every scenario: WHICH 0 gives r32 = M(pp32), 1 sv32 = M(sv)
-template <int WHICH>
+// one V-cycle for every scenario: WHICH 0 gives r32 = M(pp32), 1 sv32 = M(sv);
+// OFF: level-0 rows read the packed 16-bit column offsets
+template <int WHICH, bool OFF>
 static int bh_vcycle(BatchHost* b, const BatchDev& B, int grow) {
+  constexpr int V = WHICH | (OFF ? 2 : 0);
   isph_ctx* c = b->ctx[0];
   const int S = b->S;
   auto tot = [&](int kind) { return b->pre[(size_t)kind * (S + 1) + S]; };
-  BLAUNCH(kb_mg_pre0<WHICH>, grow, 256, B);
+  BLAUNCH(kb_mg_pre0<V>, grow, 256, B);
   BLAUNCH(kb_mg_restrict0, tot(BK_CELL1), 256, B);
   for (int l = 1; l <= b->max_nmb; l++)
     if (tot(BK_DOWN + l - 1) > 0) BLAUNCH(kb_mg_down, tot(BK_DOWN + l - 1), 256, B, l);
-  BLAUNCH(kb_mg_coarse, S, MG_CO_THREADS, B);
+  BLAUNCH_LDS(kb_mg_coarse, S, MG_CO_THREADS, b->co_lds, B);
   for (int l = b->max_nmb; l >= 1; l--)
     if (tot(BK_UP + l - 1) > 0) BLAUNCH(kb_mg_up, tot(BK_UP + l - 1), 256, B, l);
-  BLAUNCH(kb_mg_post0<WHICH>, grow, 256, B);
+  BLAUNCH(kb_mg_prolong0, grow, 256, B);
+  BLAUNCH(kb_mg_post0<V>, grow, 256, B);
+  return 0;
+}
+
+// chunk iterations of the preconditioned BiCGStab for every scenario
+template <bool OFF>
+static int bh_iterate(BatchHost* b, const BatchDev& B, int grow, int chunk) {
+  isph_ctx* c = b->ctx[0];
+  const int S = b->S;
+  int rc = 0;
+  for (int k = 0; k < chunk; k++) {
+    rc = bh_vcycle<0, OFF>(b, B, grow);
+    if (rc) return rc;
+    TIMED_BLAUNCH(2, kb_pb_1<OFF>, grow, B);
+    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_ALPHA, 0);
+    BLAUNCH(kb_pb_s, grow, 256, B);
+    rc = bh_vcycle<1, OFF>(b, B, grow);
+    if (rc) return rc;
+    TIMED_BLAUNCH(3, kb_pb_2<OFF>, grow, B);
+    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_OMEGA, 0);
+    BLAUNCH(kb_pb_3, grow, 256, B);
+    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BETA, 0);
+    BLAUNCH(kb_pb_4, grow, 256, B);
+  }
   return 0;
 }
 
@@ -927,7 +981,10 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 1);
   rc = bh_gather(b);
   if (rc) return rc;
-  std::vector<int> it_prev(S), rs_prev(S);
+  // packed 16-bit column offsets unless some scenario's rows need int32
+  bool off16 = true;
+  for (int s = 0; s < S; s++) off16 = off16 && !b->sc[s].far;
+  std::vector<int> it_prev(S), rs_prev(S), first_stop(S, -1);
   auto all_fin = [&]() {
     for (int s = 0; s < S; s++)
       if (!b->sc[s].fin) return false;
@@ -943,26 +1000,28 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   // every chunk either advances an unfinished scenario by at least one
   // iteration or ends it at the check, so this bound is never the exit
   const int64_t chunk_cap = 4 + (int64_t)(max_iter + 1) * (max_outer + 2);
+  // Chunk sizes: the iterations the scenarios needed last time to their first
+  // stop (and after it), then 2, 4, 8, 16 until they stop.  A chunk ends in a
+  // check; iterations launched after a stop exit at once.
+  auto predicted = [&](bool after) {
+    int p = 0;
+    for (int s = 0; s < S; s++)
+      if (!b->sc[s].fin) p = std::max(p, after ? b->ctx[s]->pred_after : b->ctx[s]->pred_first);
+    return p;
+  };
   int chunk = 4;
+  bool guessed = false;  // the chunk just run was sized from the last solve
+  if (const int p = predicted(false); p > 0) {
+    chunk = p;
+    guessed = true;
+  }
   for (int64_t nchunks = 0; !all_fin(); nchunks++) {
     if (nchunks > chunk_cap) {
       set_err(c, "precision solve: scenarios did not finish");
       return ISPH_ENOCONV;
     }
-    for (int k = 0; k < chunk; k++) {
-      rc = bh_vcycle<0>(b, B, grow);
-      if (rc) return rc;
-      TIMED_BLAUNCH(2, kb_pb_1, grow, B);
-      BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_ALPHA, 0);
-      BLAUNCH(kb_pb_s, grow, 256, B);
-      rc = bh_vcycle<1>(b, B, grow);
-      if (rc) return rc;
-      TIMED_BLAUNCH(3, kb_pb_2, grow, B);
-      BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_OMEGA, 0);
-      BLAUNCH(kb_pb_3, grow, 256, B);
-      BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BETA, 0);
-      BLAUNCH(kb_pb_4, grow, 256, B);
-    }
+    rc = off16 ? bh_iterate<true>(b, B, grow, chunk) : bh_iterate<false>(b, B, grow, chunk);
+    if (rc) return rc;
     // FP64 check of every scenario whose iteration stopped in this chunk
     BLAUNCH(kb_pb_resid, grow, 256, B, 2);
     BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 2);
@@ -972,12 +1031,27 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
     int worked = 0, restarted = 0;
     for (int s = 0; s < S; s++) {
       worked = std::max(worked, b->sc[s].iters - it_prev[s]);
-      restarted |= (b->sc[s].restarts != rs_prev[s]);
+      if (b->sc[s].restarts != rs_prev[s]) {
+        restarted = 1;
+        if (first_stop[s] < 0) first_stop[s] = b->sc[s].iters;
+      }
       it_prev[s] = b->sc[s].iters;
       rs_prev[s] = b->sc[s].restarts;
     }
     if (c->prof) timers_collect(c, worked);
-    chunk = restarted ? 4 : std::min(16, 2 * chunk);
+    if (restarted) {
+      const int p = predicted(true);
+      guessed = p > 0;
+      chunk = guessed ? p : 4;
+    } else {
+      chunk = guessed ? 2 : std::min(16, 2 * chunk);
+      guessed = false;
+    }
+  }
+  for (int s = 0; s < S; s++) {
+    const int tot_it = b->sc[s].iters;
+    b->ctx[s]->pred_first = (first_stop[s] >= 0) ? first_stop[s] : tot_it;
+    b->ctx[s]->pred_after = (first_stop[s] >= 0) ? tot_it - first_stop[s] : 0;
   }
   BLAUNCH(kb_copy_sol_to_p, grow, 256, B);
   for (int s = 0; s < S; s++) {

@@ -17,23 +17,23 @@
 //
 // One V-cycle M(v): x0 = w v/d; r0 = v - A x0; r1 = R r0; e1 = V-cycle on the
 // cell levels; x1 = x0 + c P e1; x = x1 + w (v - A x1)/d, with damped-Jacobi
-// weight w and coarse-correction scale c.  The fine cell levels run on many
-// workgroups (one down and one up pass per level) until the next level has
-// at most MG_MB_MIN unknowns; the remaining coarse levels and the dense solve
-// run inside one workgroup per scenario.
+// weight w and coarse-correction scale c.  The coarsest levels -- the longest
+// tail of the hierarchy whose vectors and stencils fit the workgroup's LDS
+// (mg_coarse_lds_bytes) -- and the dense solve run inside one workgroup per
+// scenario with their data in LDS; the finer cell levels run on many
+// workgroups, one down and one up pass per level.
 #pragma once
 #include "isph_device.h"
 
 #define MG_MAX_LEVELS 24
 #define MG_DENSE_MAX 64
 #define MG_CO_THREADS 1024
-// A cell level runs on many workgroups while the next level has more than
-// MG_MB_MIN unknowns (two per thread of the single coarse workgroup).  At
-// 650k particles this runs levels 1-2 multi-workgroup: 2% faster per step
-// than levels 1-3, and levels 1 only is 13% slower.
-#ifndef MG_MB_MIN
-#define MG_MB_MIN (2 * MG_CO_THREADS)
-#endif
+// The first level of the coarse workgroup has at most MG_CO_PER unknowns per
+// thread: their stencils stay in registers for the whole cycle.
+#define MG_CO_PER 3
+// LDS the coarse workgroup may use (bytes), below the device's per-workgroup
+// limit: the vectors x, b, r of its levels and the stencils below its first.
+#define MG_CO_LDS_MAX (144 * 1024)
 // Recursive-residual target of the cycles after a restart, as a fraction of
 // tol (the first cycle's fraction is the parameter inner_tol).  The last
 // cycle sets the final residual, and with it how much smooth error is left:
@@ -76,6 +76,34 @@ struct MgDev {
   float* e1;                   // final level-1 correction read by the level-0 post-smoother
 };
 
+// LDS bytes the coarse workgroup needs when it runs levels l0..nlev: the
+// dense level's inverse (FP64), x, b and r of every level, the stencils of
+// levels l0+1..nlev-1 and the diagonal of level l0 (whose stencil rows are
+// held in registers)
+__host__ __device__ inline int64_t mg_coarse_lds_bytes(const MgDev& G, int l0) {
+  int64_t v = 0, st = 0;
+  for (int l = l0; l <= G.nlev; l++) {
+    v += G.lev[l].n;
+    if (l > l0 && l < G.nlev) st += 9 * (int64_t)G.lev[l].n;
+  }
+  const int64_t nd = G.lev[G.nlev].n;
+  return 8 * nd * nd + 4 * (3 * v + st + G.lev[l0].n);
+}
+
+// neighbour k of cell (ci, cj) of level L (row offset k % 3 - 1, column
+// offset k / 3 - 1), or -1 outside the grid; k is a constant after unrolling,
+// so no division is left
+__device__ __forceinline__ int mg_nbr_at(const MgLevel& L, int ci, int cj, int k) {
+  const int a = ci + (k % 3) - 1, b = cj + (k / 3) - 1;
+  if (a < 0 || a >= L.ny || b < 0 || b >= L.nx) return -1;
+  return b * L.ny + a;
+}
+
+// the cell of level C = l+1 holding neighbour k of cell (ci, cj) of level l
+__device__ __forceinline__ int mg_parent_at(const MgLevel& C, int ci, int cj, int k) {
+  return ((cj + (k / 3) - 1) >> 1) * C.ny + ((ci + (k % 3) - 1) >> 1);
+}
+
 __device__ __forceinline__ int mg_nbr(const MgLevel& L, int I, int k) {
   const int cj = I / L.ny, ci = I - cj * L.ny;
   const int di = (k % 3) - 1, dj = (k / 3) - 1;
@@ -88,13 +116,25 @@ __device__ __forceinline__ int mg_nbr(const MgLevel& L, int I, int k) {
 // setup (once per solve)
 // ---------------------------------------------------------------------
 // level 0, row i: inverse row diagonal (diagonal plus the slot-1 value of
-// ghost rows)
+// ghost rows), and the row's packed 16-bit column offsets (a row with an
+// offset beyond 16 bits raises the scenario's far flag, and the solve reads
+// the int32 columns)
 __device__ __forceinline__ void mg_setup0_row(const Dev& D, const MgDev& G, int i) {
   if (i >= D.sc->n) return;
   const int cap = D.cap, L = D.alen[i];
   double d = 0.0;
-  for (int k = 0; k < L; k++)
-    if (D.acol[k * cap + i] == i) d = d + D.aval[k * cap + i];
+  bool far = false;
+  for (int k = 0; k < L; k += 2) {
+    const int j0 = D.acol[k * cap + i];
+    const int j1 = (k + 1 < L) ? D.acol[(k + 1) * cap + i] : i;
+    if (j0 == i) d = d + D.aval[k * cap + i];
+    if (k + 1 < L && j1 == i) d = d + D.aval[(k + 1) * cap + i];
+    const int o0 = j0 - i, o1 = j1 - i;
+    far = far || o0 < -32768 || o0 > 32767 || o1 < -32768 || o1 > 32767;
+    D.aoff[(k >> 1) * cap + i] = (uint32_t)(uint16_t)(int16_t)o0 |
+                                 ((uint32_t)(uint16_t)(int16_t)o1 << 16);
+  }
+  if (far) atomicOr(&D.sc->far, 1);
   G.d0[i] = (d != 0.0 && isfinite(d)) ? (float)(1.0 / d) : 1.0f;
 }
 
@@ -230,8 +270,10 @@ __device__ __forceinline__ void mg_dense_block(const MgDev& G) {
     }
     __syncthreads();
   }
+  // stored transposed, inv(r, cc) at [cc * n + r], so the coarse solve's
+  // threads (one per row r) read consecutive addresses
   for (int e = t; e < n * n; e += T) {
-    const int r = e / n, cc = e - r * n;
+    const int cc = e / n, r = e - cc * n;
     G.dense_inv[e] = A[r * w2 + n + cc];
   }
 }
@@ -239,20 +281,22 @@ __device__ __forceinline__ void mg_dense_block(const MgDev& G) {
 // ---------------------------------------------------------------------
 // V-cycle
 // ---------------------------------------------------------------------
-// level 0, row i: x0 = w v d ; r0 = v - A x0 (x0 of the neighbours formed on
-// the fly)
-template <typename VT>
+// level 0, row i: r0 = v - A x0, where x0 = (w v) d was written for every row
+// by the kernel that produced v (pb_s for s, pb_4 and pb_resid for p)
+template <bool OFF, typename VT>
 __device__ __forceinline__ void mg_pre0_row(const Dev& D, const MgDev& G,
                                             const VT* __restrict__ v, float* __restrict__ r0,
                                             int i) {
   if (*G.done || i >= D.sc->n) return;
-  const float w = G.w;
-  const float* __restrict__ d0 = G.d0;
-  const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
-                                              [&](int j) { return w * (float)v[j] * d0[j]; });
-  const float vi = (float)v[i];
-  G.x0[i] = w * vi * G.d0[i];
-  r0[i] = vi - acc;
+  const float* __restrict__ x0 = G.x0;
+  const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
+                                              [&](int j) { return x0[j]; });
+  r0[i] = (float)v[i] - acc;
+}
+
+// x0 = (w v) d of row i, formed by the producer of v
+__device__ __forceinline__ void mg_scale0(const MgDev& G, float v, int i) {
+  G.x0[i] = (G.w * v) * G.d0[i];
 }
 
 // level 1, cell I: r1 = R r0, the sum of the cell's contiguous row range
@@ -266,35 +310,49 @@ __device__ __forceinline__ void mg_restrict0_cell(const Dev& D, const MgDev& G,
   G.bb[L.off + I] = s;
 }
 
-// Cell level l (1 <= l <= nmb), down pass, aggregate J of level l+1: the
-// thread pre-smooths the aggregate's (up to four) cells, forms their
-// residuals with the neighbours' pre-smoothed values computed on the fly, and
-// restricts.  The arithmetic and its order are those of the single-workgroup
-// pass below, so a level gives the same values whichever pass runs it.
-__device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int J) {
+// Cell level l (1 <= l <= nmb), down pass: pre-smooth, residual, restrict.
+// Thread g works on child q = g & 3 of aggregate J = g >> 2 of level l+1
+// (cell (2 CI + (q & 1), 2 CJ + (q >> 1)) of level l): it pre-smooths the
+// cell and forms its residual with the neighbours' pre-smoothed values
+// computed on the fly; the aggregate's first lane adds the four residuals in
+// child order.  The arithmetic and its order are those of the single-
+// workgroup pass below, so a level gives the same values whichever pass runs
+// it.  The four lanes of an aggregate share a wavefront and leave together.
+__device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
   if (*G.done || l > G.nmb) return;
   const MgLevel L = G.lev[l], C = G.lev[l + 1];
+  const int J = g >> 2, q = g & 3;
   if (J >= C.n) return;
   const float w = G.w;
   const float* st = G.sten + 9 * L.off;
   const float* b = G.bb + L.off;
   float* x = G.xb + L.off;
   const int CJ = J / C.ny, CI = J - CJ * C.ny;
-  float s = 0.0f;
-  for (int bq = 0; bq < 2; bq++)
-    for (int a = 0; a < 2; a++) {
-      const int ci = 2 * CI + a, cj = 2 * CJ + bq;
-      if (ci >= L.ny || cj >= L.nx) continue;
-      const int I = cj * L.ny + ci;
-      float acc = 0.0f;
+  const int ci = 2 * CI + (q & 1), cj = 2 * CJ + (q >> 1);
+  const bool in = ci < L.ny && cj < L.nx;
+  float res = 0.0f;
+  if (in) {
+    const int I = cj * L.ny + ci;
+    float acc = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 9; k++) {
-        const int N = mg_nbr(L, I, k);
-        if (N >= 0) acc = acc + st[k * L.n + I] * (w * b[N] / st[4 * L.n + N]);
-      }
-      x[I] = w * b[I] / st[4 * L.n + I];
-      s = s + (b[I] - acc);
+    for (int k = 0; k < 9; k++) {
+      const int N = mg_nbr_at(L, ci, cj, k);
+      if (N >= 0) acc = acc + st[k * L.n + I] * (w * b[N] / st[4 * L.n + N]);
     }
+    x[I] = w * b[I] / st[4 * L.n + I];
+    res = b[I] - acc;
+  }
+  const int lane = threadIdx.x & 63, base = lane & ~3;
+  const float r1 = __shfl(res, base + 1, 64), r2 = __shfl(res, base + 2, 64),
+              r3 = __shfl(res, base + 3, 64);
+  const int in_mask = __shfl((int)in, base + 1, 64) | (__shfl((int)in, base + 2, 64) << 1) |
+                      (__shfl((int)in, base + 3, 64) << 2);
+  if (q != 0) return;
+  float s = 0.0f;
+  if (in) s = s + res;
+  if (in_mask & 1) s = s + r1;
+  if (in_mask & 2) s = s + r2;
+  if (in_mask & 4) s = s + r3;
   G.bb[C.off + J] = s;
 }
 
@@ -313,49 +371,140 @@ __device__ __forceinline__ void mg_lvl_up(const MgDev& G, int l, int I) {
   const float* b = G.bb + L.off;
   const float* x = G.xb + L.off;
   const float* __restrict__ xc = (l == G.nmb ? G.xb : G.rb) + C.off;
+  const int cj = I / L.ny, ci = I - cj * L.ny;
   float acc = 0.0f;
 #pragma unroll
   for (int k = 0; k < 9; k++) {
-    const int N = mg_nbr(L, I, k);
-    if (N >= 0) {
-      const int nj = N / L.ny, ni = N - nj * L.ny;
-      acc = acc + st[k * L.n + I] * (x[N] + c * xc[(nj >> 1) * C.ny + (ni >> 1)]);
-    }
+    const int N = mg_nbr_at(L, ci, cj, k);
+    if (N >= 0) acc = acc + st[k * L.n + I] * (x[N] + c * xc[mg_parent_at(C, ci, cj, k)]);
   }
-  const int cj = I / L.ny, ci = I - cj * L.ny;
   const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
   G.rb[L.off + I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
 }
 
-__device__ __forceinline__ float mg_apply_row(const MgLevel& L, const float* st, const float* x,
-                                              int I) {
-  float acc = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int J = mg_nbr(L, I, k);
-    if (J >= 0) acc = acc + st[k * L.n + I] * x[J];
-  }
-  return acc;
-}
+// levels l0 = nmb+1..nlev in one workgroup: down sweeps, dense solve, up
+// sweeps, with every array in LDS (mg_coarse_lds_bytes(G, l0)): the dense
+// inverse, the vectors of these levels, the stencils of levels l0+1..nlev-1
+// and level l0's diagonal; level l0's stencil rows, at most MG_CO_PER per
+// thread, stay in registers.  A down sweep is pre-smooth, residual and
+// restriction, a barrier after each; an up sweep forms the neighbours'
+// corrected values on the fly, as the multi-workgroup pass does, and writes
+// its result to the level's residual slot, one barrier per level.  The
+// arithmetic and its order are those of the multi-workgroup passes, so a
+// level gives the same values whichever pass runs it.
+#define MG_CO_STV 16  // LDS stencil floats per thread loaded in the first batch
 
-// levels nmb+1..nlev in one workgroup: down sweeps, dense solve, up sweeps
-__device__ __forceinline__ void mg_coarse_block(const MgDev& G) {
-  if (*G.done) return;
+__device__ __forceinline__ void mg_coarse_block(const MgDev& G, double* __restrict__ lds) {
   const int t = threadIdx.x, T = blockDim.x;
   const int nl = G.nlev, l0 = G.nmb + 1;
   const float w = G.w, c = G.cscale;
+  // the levels are consecutive in the pools, so the LDS image of levels
+  // l0..nl is their pool range: vectors from lev[l0].off, stencils of levels
+  // l0+1..nl-1 from 9 lev[l0+1].off
+  const MgLevel L0 = G.lev[l0], LD = G.lev[nl];
+  const int base = L0.off;
+  const int S = LD.off + LD.n - base;
+  const int stb = (l0 < nl) ? G.lev[l0 + 1].off : LD.off;
+  const int nst = 9 * (LD.off - stb);
+  const int nd2 = LD.n * LD.n;
+  double* DI = lds;                               // LD.n x LD.n, [J * n + I]
+  float* X = reinterpret_cast<float*>(lds + nd2);
+  float* Bv = X + S;
+  float* R = X + 2 * S;
+  float* ST = X + 3 * S;                          // level l's stencil at 9 (off - stb)
+  float* DG0 = ST + nst;                          // level l0's diagonal
+  // every load of the set-up is issued before the first store: one memory
+  // latency rather than one per array
+  const int dn = *G.done;
+  const float* sg0 = G.sten + 9 * L0.off;
+  const float* sgs = G.sten + 9 * stb;
+  float st0[MG_CO_PER][9], bq[MG_CO_PER];
+  double dv[(MG_DENSE_MAX * MG_DENSE_MAX + MG_CO_THREADS - 1) / MG_CO_THREADS];
+  float sv[MG_CO_STV];
+  int ci0[MG_CO_PER], cj0[MG_CO_PER];
+#pragma unroll
+  for (int q = 0; q < MG_CO_PER; q++) {
+    const int I = t + q * T;
+#pragma unroll
+    for (int k = 0; k < 9; k++) st0[q][k] = (I < L0.n) ? sg0[k * L0.n + I] : 0.0f;
+    bq[q] = (I < L0.n) ? G.bb[base + I] : 0.0f;
+  }
+#pragma unroll
+  for (int u = 0; u < (int)(sizeof dv / sizeof dv[0]); u++) {
+    const int e = t + u * T;
+    dv[u] = (e < nd2) ? G.dense_inv[e] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < MG_CO_STV; u++) {
+    const int e = t + u * T;
+    sv[u] = (e < nst) ? sgs[e] : 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < MG_CO_PER; q++) {
+    const int I = t + q * T;
+    if (I < L0.n) {
+      Bv[I] = bq[q];
+      DG0[I] = st0[q][4];
+    }
+    cj0[q] = I / L0.ny;
+    ci0[q] = I - cj0[q] * L0.ny;
+  }
+#pragma unroll
+  for (int u = 0; u < (int)(sizeof dv / sizeof dv[0]); u++) {
+    const int e = t + u * T;
+    if (e < nd2) DI[e] = dv[u];
+  }
+#pragma unroll
+  for (int u = 0; u < MG_CO_STV; u++) {
+    const int e = t + u * T;
+    if (e < nst) ST[e] = sv[u];
+  }
+  for (int e = t + MG_CO_STV * T; e < nst; e += T) ST[e] = sgs[e];
+  if (dn) return;  // the whole workgroup: the flag is per scenario
+  __syncthreads();
+  // down: x = w b / diag; r = b - A x; r restricted to the next level's b
   for (int l = l0; l < nl; l++) {
-    const MgLevel L = G.lev[l];
-    const float* st = G.sten + 9 * L.off;
-    float* x = G.xb + L.off;
-    const float* b = G.bb + L.off;
-    float* r = G.rb + L.off;
-    for (int I = t; I < L.n; I += T) x[I] = w * b[I] / st[4 * L.n + I];
+    const MgLevel L = G.lev[l], C = G.lev[l + 1];
+    const float* st = (l > l0) ? ST + 9 * (L.off - stb) : ST;  // level l0: registers
+    float* x = X + (L.off - base);
+    const float* b = Bv + (L.off - base);
+    float* r = R + (L.off - base);
+    if (l == l0) {
+#pragma unroll
+      for (int q = 0; q < MG_CO_PER; q++) {
+        const int I = t + q * T;
+        if (I < L.n) x[I] = w * b[I] / st0[q][4];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < MG_CO_PER; q++) {
+        const int I = t + q * T;
+        if (I < L.n) {
+          float acc = 0.0f;
+#pragma unroll
+          for (int k = 0; k < 9; k++) {
+            const int N = mg_nbr_at(L, ci0[q], cj0[q], k);
+            if (N >= 0) acc = acc + st0[q][k] * x[N];
+          }
+          r[I] = b[I] - acc;
+        }
+      }
+    } else {
+      for (int I = t; I < L.n; I += T) x[I] = w * b[I] / st[4 * L.n + I];
+      __syncthreads();
+      for (int I = t; I < L.n; I += T) {
+        const int cj = I / L.ny, ci = I - cj * L.ny;
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const int N = mg_nbr_at(L, ci, cj, k);
+          if (N >= 0) acc = acc + st[k * L.n + I] * x[N];
+        }
+        r[I] = b[I] - acc;
+      }
+    }
     __syncthreads();
-    for (int I = t; I < L.n; I += T) r[I] = b[I] - mg_apply_row(L, st, x, I);
-    __syncthreads();
-    const MgLevel C = G.lev[l + 1];
-    float* bc = G.bb + C.off;
+    float* bc = Bv + (C.off - base);
     for (int J = t; J < C.n; J += T) {
       const int CJ = J / C.ny, CI = J - CJ * C.ny;
       float s = 0.0f;
@@ -369,50 +518,80 @@ __device__ __forceinline__ void mg_coarse_block(const MgDev& G) {
     __syncthreads();
   }
   {
-    const MgLevel L = G.lev[nl];
-    const float* b = G.bb + L.off;
-    float* x = G.xb + L.off;
-    for (int I = t; I < L.n; I += T) {
+    const float* b = Bv + (LD.off - base);
+    float* x = X + (LD.off - base);
+    for (int I = t; I < LD.n; I += T) {
       double acc = 0.0;
-      for (int J = 0; J < L.n; J++) acc = acc + G.dense_inv[I * L.n + J] * (double)b[J];
+#pragma unroll 8
+      for (int J = 0; J < LD.n; J++) acc = acc + DI[J * LD.n + I] * (double)b[J];
       x[I] = (float)acc;
     }
     __syncthreads();
   }
+  // up: x1 = x + c P xc (the neighbours' x1 on the fly); the result
+  // x1 + w (b - A x1) / diag goes to the level's residual slot.  xc is the
+  // dense solution for the level above the dense one, else that level's
+  // residual slot.
   for (int l = nl - 1; l >= l0; l--) {
     const MgLevel L = G.lev[l], C = G.lev[l + 1];
-    const float* st = G.sten + 9 * L.off;
-    float* x = G.xb + L.off;
-    const float* b = G.bb + L.off;
-    const float* xc = G.xb + C.off;
-    for (int I = t; I < L.n; I += T) {
-      const int cj = I / L.ny, ci = I - cj * L.ny;
-      x[I] = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+    const float* st = (l > l0) ? ST + 9 * (L.off - stb) : ST;  // level l0: registers
+    const float* x = X + (L.off - base);
+    const float* b = Bv + (L.off - base);
+    const float* xc = (l + 1 == nl ? X : R) + (C.off - base);
+    float* out = R + (L.off - base);
+    if (l == l0) {
+#pragma unroll
+      for (int q = 0; q < MG_CO_PER; q++) {
+        const int I = t + q * T;
+        if (I < L.n) {
+          const int ci = ci0[q], cj = cj0[q];
+          float acc = 0.0f;
+#pragma unroll
+          for (int k = 0; k < 9; k++) {
+            const int N = mg_nbr_at(L, ci, cj, k);
+            if (N >= 0) acc = acc + st0[q][k] * (x[N] + c * xc[mg_parent_at(C, ci, cj, k)]);
+          }
+          const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+          out[I] = x1 + w * (b[I] - acc) / st0[q][4];
+        }
+      }
+    } else {
+      for (int I = t; I < L.n; I += T) {
+        const int cj = I / L.ny, ci = I - cj * L.ny;
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const int N = mg_nbr_at(L, ci, cj, k);
+          if (N >= 0) acc = acc + st[k * L.n + I] * (x[N] + c * xc[mg_parent_at(C, ci, cj, k)]);
+        }
+        const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+        out[I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+      }
     }
     __syncthreads();
-    float* r = G.rb + L.off;
-    for (int I = t; I < L.n; I += T) r[I] = b[I] - mg_apply_row(L, st, x, I);
-    __syncthreads();
-    for (int I = t; I < L.n; I += T) x[I] = x[I] + w * r[I] / st[4 * L.n + I];
-    __syncthreads();
   }
+  // level l0's result for the finer passes
+  const float* res = (l0 == nl ? X : R);
+  for (int I = t; I < L0.n; I += T) G.xb[L0.off + I] = res[I];
 }
 
-// level 0, row i: x = x1 + w (v - A x1) d with x1 = x0 + c P e1 (x1 of the
-// neighbours formed on the fly)
-template <typename VT>
+// level 0, row i: x1 = x0 + c P e1, in place of x0 (e1 is the final level-1
+// correction of the row's cell)
+__device__ __forceinline__ void mg_prolong0_row(const Dev& D, const MgDev& G, int i) {
+  if (*G.done || i >= D.sc->n) return;
+  G.x0[i] = G.x0[i] + G.cscale * G.e1[D.cell[i]];
+}
+
+// level 0, row i: x = x1 + w (v - A x1) d, x1 in x0's slot
+template <bool OFF, typename VT>
 __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
                                              const VT* __restrict__ v, float* __restrict__ out,
                                              int i) {
   if (*G.done || i >= D.sc->n) return;
-  const float c = G.cscale;
-  const float* __restrict__ e1 = G.e1;
-  const float* __restrict__ x0 = G.x0;
-  const int32_t* __restrict__ cell = D.cell;
-  const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
-                                              [&](int j) { return x0[j] + c * e1[cell[j]]; });
-  const float x1 = G.x0[i] + c * e1[D.cell[i]];
-  out[i] = x1 + G.w * ((float)v[i] - acc) * G.d0[i];
+  const float* __restrict__ x1 = G.x0;
+  const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
+                                              [&](int j) { return x1[j]; });
+  out[i] = x1[i] + G.w * ((float)v[i] - acc) * G.d0[i];
 }
 
 // ---------------------------------------------------------------------
@@ -422,6 +601,7 @@ __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
 // the scenario's scalars, so it is taken by whole workgroups.
 // ---------------------------------------------------------------------
 // t1 = A32 ph ; partial (t1 . rh)
+template <bool OFF>
 __device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
   if (D.sc->done) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
@@ -429,7 +609,7 @@ __device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
   double prod = 0.0;
   if (i < n) {
     const float* __restrict__ ph = D.r32;
-    const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
+    const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
                                                 [&](int j) { return ph[j]; });
     D.t132[i] = acc;
     prod = (double)acc * (double)D.rh32[i];
@@ -439,14 +619,17 @@ __device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
 }
 
 // s = r - alpha t1 (FP64)
-__device__ __forceinline__ void pb_s_rows(const Dev& D, int vb) {
+__device__ __forceinline__ void pb_s_rows(const Dev& D, const MgDev& G, int vb) {
   if (D.sc->done) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
-  D.sv[i] = D.r[i] - D.sc->alpha * (double)D.t132[i];
+  const double sv = D.r[i] - D.sc->alpha * (double)D.t132[i];
+  D.sv[i] = sv;
+  mg_scale0(G, (float)sv, i);
 }
 
 // t2 = A32 sh ; partials (t2 . s), (t2 . t2)
+template <bool OFF>
 __device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
   if (D.sc->done) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
@@ -454,7 +637,7 @@ __device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
     const float* __restrict__ sh = D.sv32;
-    const float acc = ell_row_sum<ELL_U, float>(D.acol, D.aval32, D.cap, i, D.alen[i],
+    const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
                                                 [&](int j) { return sh[j]; });
     D.t232[i] = acc;
     p1 = (double)acc * D.sv[i];
@@ -491,12 +674,14 @@ __device__ __forceinline__ void pb_3_rows(const Dev& D, int vb, double* lds) {
 }
 
 // p = r + beta (p - omega t1)  (p stored FP32)
-__device__ __forceinline__ void pb_4_rows(const Dev& D, int vb) {
+__device__ __forceinline__ void pb_4_rows(const Dev& D, const MgDev& G, int vb) {
   if (D.sc->done) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
   const double beta = D.sc->beta, omega = D.sc->omega;
-  D.pp32[i] = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * (double)D.t132[i]));
+  const float p = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * (double)D.t132[i]));
+  D.pp32[i] = p;
+  mg_scale0(G, p, i);
 }
 
 // r = b - A64 x (FP64 true residual), and a fresh start of the Krylov space
@@ -505,7 +690,8 @@ __device__ __forceinline__ void pb_4_rows(const Dev& D, int vb) {
 //           (finished by the norm) gets x = 0 instead.
 // init = 2: check of a scenario whose iteration stopped (done, not final);
 //           if the check fails the iteration restarts from this state.
-__device__ __forceinline__ void pb_resid_rows(const Dev& D, int vb, int init, double* lds) {
+__device__ __forceinline__ void pb_resid_rows(const Dev& D, const MgDev& G, int vb, int init,
+                                              double* lds) {
   const DevScalars* S = D.sc;
   const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = S->n;
@@ -523,6 +709,7 @@ __device__ __forceinline__ void pb_resid_rows(const Dev& D, int vb, int init, do
     D.r[i] = r;
     D.rh32[i] = (float)r;
     D.pp32[i] = (float)r;
+    mg_scale0(G, (float)r, i);
     p1 = r * (double)(float)r;
     p2 = r * r;
   }

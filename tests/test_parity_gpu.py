@@ -235,3 +235,38 @@ def test_precision_form_residual_and_pressure_error(pkg, oracle, cases):
     print(f"precision form: {st.iterations} iterations, rel residual {st.rel_residual:.2e}; "
           f"FP64 reference {st64.iterations} iterations; pressure error {rel:.2e}")
     assert rel <= 1e-6  # the precision form's pressure target
+
+
+def test_precision_int32_column_path_matches_offsets(pkg, cases):
+    """The precision form's FP32 row kernels read 16-bit column offsets j - i
+    when every offset of the rows fits, else the int32 columns.  Appending to
+    row 0 an explicit zero entry in the last column (38k rows away) forces the
+    int32 path without changing any sum: iterations and solution must equal
+    the offset path's bit for bit."""
+    c = cases.porous_flume(dx=0.005, length=3.0, depth=0.3, tank_h=0.4, toe=2.0, slope=0.1)
+    assert c.n > 32768 + 1000
+    prm = dict(c.params)
+    prm.update(precision=1, tol=1e-8)
+    g = pkg.Scenario(c, params=prm)
+    g.init_state()
+    g.step(1)
+    g.ppe_assemble()
+    ln, col, val, rhs = g.ppe()
+    x0 = g.solution()
+    g.load_system(ln, col, val, rhs, x0=x0)
+    st_a = g.ppe_solve()
+    xa = g.solution()
+    k = int(ln[0])
+    assert col[0, k - 1] < c.n - 1
+    ln2, col2, val2 = ln.copy(), col.copy(), val.copy()
+    col2[0, k] = c.n - 1
+    val2[0, k] = 0.0
+    ln2[0] = k + 1
+    g.load_system(ln2, col2, val2, rhs, x0=x0)
+    st_b = g.ppe_solve()
+    xb = g.solution()
+    print(f"n={c.n}: offsets {st_a.iterations} iterations, int32 columns {st_b.iterations}",
+          flush=True)
+    assert st_a.converged and st_b.converged
+    assert (st_a.iterations, st_a.outer) == (st_b.iterations, st_b.outer)
+    assert np.array_equal(xa, xb)
