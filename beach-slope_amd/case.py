@@ -66,7 +66,7 @@ def default_params(dx: float, h_fac: float = 1.3) -> dict:
         g=G, mu=0.0010, por=0.380, d50=0.001020, lamk=600.0, csh=0.50,
         maxshift=0.1, fs_thresh=0.80 * 2.0, sig1=0.20, tschmidt=1.0,
         xlcutoff=-1.0e30, ytcutoff=-1.0e30, buf_xmin=1.0e30, buf_xmax=-1.0e30,
-        dt=1.0e-4, tol=1.0e-10, inner_tol=1.0,
+        dt=1.0e-4, tol=1.0e-10, inner_tol=0.0,
     )
 
 

@@ -86,7 +86,7 @@ struct Dev {
   uint32_t* aoff;   // precision form: column offsets j - i as 16-bit pairs,
                     // slots 2w and 2w+1 of row i in word [w * cap + i]
   double *r, *rh, *pp, *t1, *sv, *t2, *part;
-  float *r32, *rh32, *pp32, *t132, *sv32, *t232, *d32;
+  float *r32, *rh32, *pp32, *sv32, *d32;
 };
 
 // ---------------------------------------------------------------------

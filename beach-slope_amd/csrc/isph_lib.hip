@@ -163,7 +163,7 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   per += 6 * 8;                                 // FP64 solver vectors
   // precision form: FP32 matrix, packed column offsets, FP32 vectors, level-0
   // multigrid vectors, setup rows
-  if (precision) per += ROW_CAP * 4 + 4 * ((ROW_CAP + 1) / 2) + 7 * 4 + 2 * 4 + 9 * 8;
+  if (precision) per += ROW_CAP * 4 + 4 * ((ROW_CAP + 1) / 2) + 5 * 4 + 2 * 4 + 9 * 8;
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
   b += 8 * 2 * (n_cap / 256 + 2);               // reduction partials
   if (precision) b += 4 * 12 * ncell + 8 * MG_DENSE_MAX * MG_DENSE_MAX;  // cell levels
@@ -234,7 +234,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   if (params->precision) {
     rc |= dalloc(c, &D.aval32, N * ROW_CAP);
     rc |= dalloc(c, &D.aoff, N * ((ROW_CAP + 1) / 2));
-    float** ff[] = {&D.r32, &D.rh32, &D.pp32, &D.t132, &D.sv32, &D.t232, &D.d32};
+    float** ff[] = {&D.r32, &D.rh32, &D.pp32, &D.sv32, &D.d32};
     for (auto p : ff) rc |= dalloc(c, p, N);
     // cell hierarchy: level 1 is the cell grid, then 2x2 aggregation down to
     // at most MG_DENSE_MAX unknowns
@@ -954,8 +954,9 @@ static int bh_iterate(BatchHost* b, const BatchDev& B, int grow, int chunk) {
 }
 
 // Precision-form solve of every scenario of b: cell-multigrid right-
-// preconditioned BiCGStab with FP32 matrix, V-cycles and search directions;
-// FP64 solution, residual and dots.  A recursive-residual convergence claim
+// preconditioned BiCGStab with an FP32 V-cycle (matrix copy and vectors) and
+// FP32 search directions; FP64 outer products A ph and A sh, solution,
+// residual and dots.  A recursive-residual convergence claim
 // is checked against b - A64 x after the chunk in which it was made; when the
 // check fails the iteration continues from the FP64 residual.  The host only
 // polls whether every scenario has finished; stats[s] gets scenario s's.

@@ -34,13 +34,14 @@
 // LDS the coarse workgroup may use (bytes), below the device's per-workgroup
 // limit: the vectors x, b, r of its levels and the stencils below its first.
 #define MG_CO_LDS_MAX (144 * 1024)
-// Recursive-residual target of the cycles after a restart, as a fraction of
-// tol (the first cycle's fraction is the parameter inner_tol).  The last
-// cycle sets the final residual, and with it how much smooth error is left:
-// on the jittered dam break's same-system check the pressure error is
-// 4.0e-6 at 1.0 and 2.1e-7 at 0.25 (against a 1e-13 FP64 solve).
-#ifndef PB_LATER_FRAC
-#define PB_LATER_FRAC 0.25
+// Recursive-residual target of every cycle as a fraction of tol, unless the
+// parameter inner_tol sets it.  The residual left sets how much smooth error
+// stays: on the jittered dam break (7,485 particles) the same-system pressure
+// error against a 1e-13 FP64 solve is 1.0e-5 at 1, 2.9e-6 at 0.25, 1.4e-6 at
+// 0.1, 3.1e-7 at 0.05 and 6.5e-8 at 0.02; at 650k, 0.05 costs 4% per step
+// over 1.
+#ifndef PB_STOP_FRAC
+#define PB_STOP_FRAC 0.05
 #endif
 
 // Damped-Jacobi weight and coarse-correction scale of the V-cycle, from a
@@ -596,11 +597,15 @@ __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
 
 // ---------------------------------------------------------------------
 // preconditioned mixed-precision BiCGStab, row bodies (row block vb).
-// ph = M(p) lives in r32, sh = M(s) in sv32.  Blocks write partial dot
+// ph = M(p) lives in r32, sh = M(s) in sv32 (FP32 directions).  The outer
+// products t1 = A ph and t2 = A sh take the FP64 matrix values and sum in
+// FP64, so the recursive residual follows the true one (b - A x changes by
+// exactly what the recursion subtracts, up to FP64 rounding) and the FP64
+// check at the stop passes without a restart.  Blocks write partial dot
 // products at vb and nblk + vb.  Every early return below depends only on
 // the scenario's scalars, so it is taken by whole workgroups.
 // ---------------------------------------------------------------------
-// t1 = A32 ph ; partial (t1 . rh)
+// t1 = A ph (FP64) ; partial (t1 . rh)
 template <bool OFF>
 __device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
   if (D.sc->done) return;
@@ -609,10 +614,10 @@ __device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
   double prod = 0.0;
   if (i < n) {
     const float* __restrict__ ph = D.r32;
-    const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
-                                                [&](int j) { return ph[j]; });
-    D.t132[i] = acc;
-    prod = (double)acc * (double)D.rh32[i];
+    const double acc = ell_row_sum_any<OFF, ELL_UP, double>(D, D.aval, i,
+                                                 [&](int j) { return (double)ph[j]; });
+    D.t1[i] = acc;
+    prod = acc * (double)D.rh32[i];
   }
   const double s = block_tree(prod, lds);
   if (threadIdx.x == 0) D.part[vb] = s;
@@ -623,12 +628,12 @@ __device__ __forceinline__ void pb_s_rows(const Dev& D, const MgDev& G, int vb) 
   if (D.sc->done) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
-  const double sv = D.r[i] - D.sc->alpha * (double)D.t132[i];
+  const double sv = D.r[i] - D.sc->alpha * D.t1[i];
   D.sv[i] = sv;
   mg_scale0(G, (float)sv, i);
 }
 
-// t2 = A32 sh ; partials (t2 . s), (t2 . t2)
+// t2 = A sh (FP64) ; partials (t2 . s), (t2 . t2)
 template <bool OFF>
 __device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
   if (D.sc->done) return;
@@ -637,11 +642,11 @@ __device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
     const float* __restrict__ sh = D.sv32;
-    const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
-                                                [&](int j) { return sh[j]; });
-    D.t232[i] = acc;
-    p1 = (double)acc * D.sv[i];
-    p2 = (double)acc * (double)acc;
+    const double acc = ell_row_sum_any<OFF, ELL_UP, double>(D, D.aval, i,
+                                                 [&](int j) { return (double)sh[j]; });
+    D.t2[i] = acc;
+    p1 = acc * D.sv[i];
+    p2 = acc * acc;
   }
   const double s1 = block_tree(p1, lds);
   const double s2 = block_tree(p2, lds);
@@ -660,7 +665,7 @@ __device__ __forceinline__ void pb_3_rows(const Dev& D, int vb, double* lds) {
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
     D.sol[i] = (D.sol[i] + alpha * (double)D.r32[i]) + omega * (double)D.sv32[i];
-    const double r = D.sv[i] - omega * (double)D.t232[i];
+    const double r = D.sv[i] - omega * D.t2[i];
     D.r[i] = r;
     p1 = r * (double)D.rh32[i];
     p2 = r * r;
@@ -679,7 +684,7 @@ __device__ __forceinline__ void pb_4_rows(const Dev& D, const MgDev& G, int vb) 
   const int i = vb * RED_BLOCK + threadIdx.x;
   if (i >= D.sc->n) return;
   const double beta = D.sc->beta, omega = D.sc->omega;
-  const float p = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * (double)D.t132[i]));
+  const float p = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * D.t1[i]));
   D.pp32[i] = p;
   mg_scale0(G, p, i);
 }
@@ -735,14 +740,11 @@ __device__ __forceinline__ void pb_bnorm_rows(const Dev& D, int vb, double* lds)
 //               scenario (converged with x = 0 for a zero right-hand side)
 //   PFIN_ALPHA / PFIN_OMEGA / PFIN_BETA  the BiCGStab scalars; PFIN_BETA
 //               stops on the recursive residual at frac * tol only as a
-//               candidate (first cycle: frac = inner_tol, 1 when unset; later
-//               cycles: PB_LATER_FRAC).  The FP32 products keep the recursive
-//               residual from following the true one below about
-//               eps32 ||A|| ||x||, so the first check fails and a restart from
-//               the FP64 residual finishes the solve; stopping the first cycle
-//               at tol itself wastes the fewest iterations (at 650k with the
-//               earlier V-cycle constants: 62 per solve at 1, 68 at 0.25, 81
-//               at 0.1)
+//               candidate (frac = inner_tol, PB_STOP_FRAC when unset), which
+//               the FP64 check confirms.  With
+//               the outer products in FP64 the recursive residual follows the
+//               true one, and at 650k the first check passes: 38 iterations
+//               per solve, where FP32 outer products needed 57 and a restart.
 //   PFIN_RESID  init 1: start state from the FP64 residual;
 //               init 2: check of a stopped iteration against the FP64
 //               residual: met -> final; else, unless the iteration or restart
@@ -794,8 +796,7 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, doub
       S->iters = S->iters + 1;
       S->rr = rr;
       S->rel = sqrt(rr) / S->bnorm;
-      const double frac = (S->restarts == 0) ? ((P.inner_tol > 0.0) ? P.inner_tol : 1.0)
-                                             : PB_LATER_FRAC;
+      const double frac = (P.inner_tol > 0.0) ? P.inner_tol : PB_STOP_FRAC;
       if (sqrt(rr) <= frac * P.tol * S->bnorm) {
         S->conv = 1;  // candidate: confirmed against the FP64 residual (PFIN_RESID, init 2)
         S->done = 1;

@@ -10,9 +10,9 @@ device resident before the timed region starts.
 
 Two forms are measured on the same input:
   plain   FP64 BiCGStab (the parity path);
-  chosen  BiCGStab right-preconditioned by a cell-multigrid V-cycle, FP32
-          matrix, V-cycle and directions, FP64 solution, residual and dots,
-          with a final FP64 residual check.
+  chosen  BiCGStab right-preconditioned by a cell-multigrid V-cycle: FP32
+          V-cycle and search directions, FP64 outer products, solution,
+          residual and dots, with a final FP64 residual check.
 Both solve to the same relative residual (1e-8) so their throughputs compare
 like for like.  `value` is the chosen form.
 
@@ -61,7 +61,7 @@ METRIC = "particle-steps/s at 650k ISPH; scenarios/h + p99 latency vs plain GPU 
 
 def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
     params = dict(case.params)
-    params.update(precision=precision, tol=tol, inner_tol=1.0, max_outer=20)
+    params.update(precision=precision, tol=tol, max_outer=20)
     g = pkg.Scenario(case, params=params)
     g.init_state()
     for _ in range(warmup):

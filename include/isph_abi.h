@@ -59,8 +59,10 @@ enum { ISPH_F_BUFFER = 1, ISPH_F_FREE = 2, ISPH_F_VICINITY = 4 };
  *   ISPH_PPE_FP64        BiCGStab in FP64, iterates identical to the CPU
  *                        restatement (the parity path).
  *   ISPH_PPE_FP32_REFINE BiCGStab right-preconditioned by a cell-aggregation
- *                        multigrid V-cycle; FP32 matrix, V-cycle and search
- *                        directions, FP64 solution, residual and dot products;
+ *                        multigrid V-cycle; FP32 V-cycle (matrix copy and
+ *                        vectors) and search directions; FP64 products of the
+ *                        matrix with the directions, solution, residual and
+ *                        dot products;
  *                        a convergence claim is checked against the FP64
  *                        residual b - A x and the iteration restarts from it
  *                        when the check fails (iterative refinement). */
@@ -100,7 +102,7 @@ typedef struct isph_params {
   double dt;              /* initial (or fixed) time step                      */
   double tol;             /* PPE relative residual target ||b-Ax||/||b||       */
   double inner_tol;       /* precision form: the recursive residual stops at
-                             inner_tol * tol (0: 1.0) before the FP64 check   */
+                             inner_tol * tol (0: 0.05) before the FP64 check  */
 } isph_params;
 
 /* Statistics of the last PPE solve. */
