@@ -55,7 +55,7 @@ program isph_main
   integer :: ia
 
   if (command_argument_count() < 2) then
-    write(*, '(a)') 'usage: isph_driver <case.bin> <state_out.bin> [fused]'
+    write(*, '(a)') 'usage: isph_driver <case.bin> <state_out.bin> [fused] [snapshots=DIR]'
     stop 2
   end if
   call get_command_argument(1, inpath)

@@ -269,8 +269,9 @@ int isph_batch_destroy(isph_batch *b);
 /* ---- kernel timing (HIP events on the context's stream) ----
  * When enabled, every launch of the solver's two SpMV kernels is bracketed by
  * a pair of HIP events; isph_prof_read returns the summed event time and the
- * launch count per kernel.  kernel: "spmv_p" (t = A p), "spmv_s" (t = A s),
- * "spmv_p32", "spmv_s32" (the FP32 inner solve). */
+ * launch count per kernel.  kernel: "spmv_p" (t = A p), "spmv_s" (t = A s)
+ * of the FP64 form; "spmv_p32", "spmv_s32" (t1 = A ph, t2 = A sh) of the
+ * precision form. */
 int isph_prof_enable(isph_ctx *ctx, int32_t on);
 int isph_prof_read(isph_ctx *ctx, const char *kernel, double *total_ms,
                    int64_t *launches);
