@@ -102,15 +102,11 @@ __global__ void __launch_bounds__(256) kb_mg_down(BatchDev B, int l) {
 }
 __global__ void __launch_bounds__(MG_CO_THREADS) kb_mg_coarse(BatchDev B) {
   extern __shared__ double mg_lds[];
-  mg_coarse_block(B.mgs[blockIdx.x], mg_lds);
+  mg_coarse_block(B.devs[blockIdx.x], B.mgs[blockIdx.x], mg_lds);
 }
 __global__ void __launch_bounds__(256) kb_mg_up(BatchDev B, int l) {
   BATCH_CELLS(B, BK_UP + l - 1);
-  mg_lvl_up(G, l, I_);
-}
-__global__ void __launch_bounds__(256) kb_mg_prolong0(BatchDev B) {
-  BATCH_ROW(B);
-  mg_prolong0_row(D, B.mgs[s_], vb_ * RED_BLOCK + threadIdx.x);
+  mg_lvl_up(D, G, l, I_);
 }
 template <int V>
 __global__ void __launch_bounds__(256) kb_mg_post0(BatchDev B) {

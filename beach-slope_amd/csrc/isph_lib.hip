@@ -290,8 +290,6 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
       l0++;
     G.nmb = l0 - 1;
     c->co_lds = (int)mg_coarse_lds_bytes(G, l0);
-    // level 1's final correction: rb when level 1 runs multi-block, else xb
-    G.e1 = (G.nmb >= 1) ? G.rb + G.lev[1].off : G.xb + G.lev[1].off;
   }
   DevScalars s{};
   s.dt = params->dt;
@@ -925,7 +923,6 @@ static int bh_vcycle(BatchHost* b, const BatchDev& B, int grow) {
   BLAUNCH_LDS(kb_mg_coarse, S, MG_CO_THREADS, b->co_lds, B);
   for (int l = b->max_nmb; l >= 1; l--)
     if (tot(BK_UP + l - 1) > 0) BLAUNCH(kb_mg_up, tot(BK_UP + l - 1), 256, B, l);
-  BLAUNCH(kb_mg_prolong0, grow, 256, B);
   BLAUNCH(kb_mg_post0<V>, grow, 256, B);
   return 0;
 }

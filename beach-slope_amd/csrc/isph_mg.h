@@ -74,7 +74,6 @@ struct MgDev {
   float* x0;                   // level 0 pre-smoothed iterate
   float w, cscale;
   const int* done;             // solver's done flag: V-cycles after convergence are no-ops
-  float* e1;                   // final level-1 correction read by the level-0 post-smoother
 };
 
 // LDS bytes the coarse workgroup needs when it runs levels l0..nlev: the
@@ -311,6 +310,22 @@ __device__ __forceinline__ void mg_restrict0_cell(const Dev& D, const MgDev& G,
   G.bb[L.off + I] = s;
 }
 
+// level 0 rows of level-1 cell I: x1 = x0 + c e1, in place of x0 (e1 is the
+// cell's final level-1 correction); run by the pass that forms e1
+__device__ __forceinline__ void mg_prolong_cell(const Dev& D, const MgDev& G, int I, float e1) {
+  const float ce = G.cscale * e1;
+  float* __restrict__ x0 = G.x0;
+  const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
+  for (int b0 = i0; b0 < i1; b0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = (b0 + u < i1) ? x0[b0 + u] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (b0 + u < i1) x0[b0 + u] = v[u] + ce;
+  }
+}
+
 // Cell level l (1 <= l <= nmb), down pass: pre-smooth, residual, restrict.
 // Thread g works on child q = g & 3 of aggregate J = g >> 2 of level l+1
 // (cell (2 CI + (q & 1), 2 CJ + (q >> 1)) of level l): it pre-smooths the
@@ -361,9 +376,9 @@ __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
 // level-(l+1) correction and post-smooths, with the neighbours' corrected
 // values formed on the fly.  The level-(l+1) correction is in xb when that
 // level ran in the single workgroup (l == nmb), else in rb.  The result goes
-// to the level's residual slot (free by then), where the next finer level,
-// and for level 1 the particle level (e1), read it.
-__device__ __forceinline__ void mg_lvl_up(const MgDev& G, int l, int I) {
+// to the level's residual slot (free by then), where the next finer level
+// reads it; on level 1 the lane also adds it to the x0 of its cell's rows.
+__device__ __forceinline__ void mg_lvl_up(const Dev& D, const MgDev& G, int l, int I) {
   if (*G.done || l > G.nmb) return;
   const MgLevel L = G.lev[l], C = G.lev[l + 1];
   if (I >= L.n) return;
@@ -380,7 +395,9 @@ __device__ __forceinline__ void mg_lvl_up(const MgDev& G, int l, int I) {
     if (N >= 0) acc = acc + st[k * L.n + I] * (x[N] + c * xc[mg_parent_at(C, ci, cj, k)]);
   }
   const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
-  G.rb[L.off + I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+  const float out = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+  G.rb[L.off + I] = out;
+  if (l == 1) mg_prolong_cell(D, G, I, out);
 }
 
 // levels l0 = nmb+1..nlev in one workgroup: down sweeps, dense solve, up
@@ -395,7 +412,8 @@ __device__ __forceinline__ void mg_lvl_up(const MgDev& G, int l, int I) {
 // level gives the same values whichever pass runs it.
 #define MG_CO_STV 16  // LDS stencil floats per thread loaded in the first batch
 
-__device__ __forceinline__ void mg_coarse_block(const MgDev& G, double* __restrict__ lds) {
+__device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
+                                                double* __restrict__ lds) {
   const int t = threadIdx.x, T = blockDim.x;
   const int nl = G.nlev, l0 = G.nmb + 1;
   const float w = G.w, c = G.cscale;
@@ -571,16 +589,13 @@ __device__ __forceinline__ void mg_coarse_block(const MgDev& G, double* __restri
     }
     __syncthreads();
   }
-  // level l0's result for the finer passes
+  // level l0's result for the finer passes, or, when l0 is level 1, added to
+  // the x0 of each cell's rows
   const float* res = (l0 == nl ? X : R);
-  for (int I = t; I < L0.n; I += T) G.xb[L0.off + I] = res[I];
-}
-
-// level 0, row i: x1 = x0 + c P e1, in place of x0 (e1 is the final level-1
-// correction of the row's cell)
-__device__ __forceinline__ void mg_prolong0_row(const Dev& D, const MgDev& G, int i) {
-  if (*G.done || i >= D.sc->n) return;
-  G.x0[i] = G.x0[i] + G.cscale * G.e1[D.cell[i]];
+  for (int I = t; I < L0.n; I += T) {
+    G.xb[L0.off + I] = res[I];
+    if (l0 == 1) mg_prolong_cell(D, G, I, res[I]);
+  }
 }
 
 // level 0, row i: x = x1 + w (v - A x1) d, x1 in x0's slot
