@@ -223,9 +223,13 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
   const int q = quadrant(xi, yi, xl, xr, yb, yt);
   const int r0 = ci - ((q == 2 || q == 3) ? 1 : 0);
   const int c0 = cj - ((q == 0 || q == 3) ? 1 : 0);
-  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
   int cnt = 0;
   const int cap = D.cap;
+  // Two passes, so the FP64 work of the accepted pairs runs with the lanes of
+  // a wavefront together rather than under the candidate test's divergence:
+  // the candidates are tested and the list written first, then the list is
+  // walked in its order.  r is recomputed from the same operands, so every
+  // value and sum is the one of a single pass.
 #pragma unroll 1
   for (int half = 0; half < 2; half++) {
     const int k0 = (c0 + half) * P.celly + r0;
@@ -237,23 +241,30 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
       const double dx = xi - xj, dy = yi - yj;
       const double d2 = (dx * dx) + (dy * dy);
       if (d2 > P.lim2) continue;
-      const double r = sqrt(d2);
-      if (!(r < P.two_h)) continue;
-      const double k1 = d_wabd(dx, r, P), k2 = d_wabd(dy, r, P);
+      if (!(sqrt(d2) < P.two_h)) continue;
       if (cnt < NBR_CAP) D.nbr[cnt * cap + i] = j;
       cnt++;
-      const double mj = D.m[j], rj = D.rho[j];
-      const double r1 = (mj * (xj - xi)) / rj;
-      const double r3 = (mj * (yj - yi)) / rj;
-      m0 = m0 + k1 * r1;
-      m1 = m1 + k2 * r1;
-      m2 = m2 + k1 * r3;
-      m3 = m3 + k2 * r3;
     }
   }
   if (cnt > NBR_CAP) {
+    // the step fails with ISPH_ECAP; the sums below cover the stored entries
     atomicOr(&D.sc->err, ERR_NBR_CAP);
     cnt = NBR_CAP;
+  }
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  for (int mm = 0; mm < cnt; mm++) {
+    const int j = D.nbr[mm * cap + i];
+    const double xj = D.x[j], yj = D.y[j];
+    const double dx = xi - xj, dy = yi - yj;
+    const double r = sqrt((dx * dx) + (dy * dy));
+    const double k1 = d_wabd(dx, r, P), k2 = d_wabd(dy, r, P);
+    const double mj = D.m[j], rj = D.rho[j];
+    const double r1 = (mj * (xj - xi)) / rj;
+    const double r3 = (mj * (yj - yi)) / rj;
+    m0 = m0 + k1 * r1;
+    m1 = m1 + k2 * r1;
+    m2 = m2 + k1 * r3;
+    m3 = m3 + k2 * r3;
   }
   D.ncnt[i] = cnt;
   D.xnb[i] = xi;
