@@ -15,6 +15,7 @@
 // -ffp-contract=off so no FMA is formed and every rounding happens where the
 // Fortran rounds.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -82,7 +83,8 @@ struct Dev {
   // PPE rows and solver vectors
   int32_t *alen, *acol;
   double *aval, *b, *sol;
-  float* aval32;
+  uint32_t* av16;   // precision form: FP16 values of the V-cycle's level-0 matrix,
+                    // slots 2w and 2w+1 of row i in word [w * cap + i] (from setup)
   uint32_t* aoff;   // precision form: column offsets j - i as 16-bit pairs,
                     // slots 2w and 2w+1 of row i in word [w * cap + i]
   double *r, *rh, *pp, *t1, *sv, *t2, *part;
@@ -156,6 +158,41 @@ __device__ __forceinline__ A ell_row_sum_off(const uint32_t* __restrict__ off,
     A g[U];
 #pragma unroll
     for (int u = 0; u < U; u++) g[u] = (k0 + u < L) ? gather(c[u]) : (A)0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u < L) acc = acc + v[u] * g[u];
+  }
+  return acc;
+}
+
+// Row sum for the V-cycle's level-0 matrix: FP16 values two to a word, and
+// the packed 16-bit column offsets (OFF) or the int32 columns; the products
+// are summed in FP32 in entry order.  U is even.
+template <bool OFF, int U, typename G>
+__device__ __forceinline__ float ell_row_sum_h(const Dev& D, int i, G gather) {
+  const int cap = D.cap, L = D.alen[i];
+  float acc = 0.0f;
+  for (int k0 = 0; k0 < L; k0 += U) {
+    int c[U];
+    float v[U];
+#pragma unroll
+    for (int q = 0; q < U / 2; q++) {
+      const bool in = k0 + 2 * q < L;
+      const uint32_t hv = in ? D.av16[((k0 >> 1) + q) * cap + i] : 0u;
+      v[2 * q] = __half2float(__ushort_as_half((unsigned short)(hv & 0xffffu)));
+      v[2 * q + 1] = __half2float(__ushort_as_half((unsigned short)(hv >> 16)));
+      if (OFF) {
+        const uint32_t wd = in ? D.aoff[((k0 >> 1) + q) * cap + i] : 0u;
+        c[2 * q] = i + (int)(int16_t)(wd & 0xffffu);
+        c[2 * q + 1] = i + (int)(int16_t)(wd >> 16);
+      } else {
+        c[2 * q] = in ? D.acol[(k0 + 2 * q) * cap + i] : i;
+        c[2 * q + 1] = (k0 + 2 * q + 1 < L) ? D.acol[(k0 + 2 * q + 1) * cap + i] : i;
+      }
+    }
+    float g[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) g[u] = (k0 + u < L) ? gather(c[u]) : 0.0f;
 #pragma unroll
     for (int u = 0; u < U; u++)
       if (k0 + u < L) acc = acc + v[u] * g[u];

@@ -477,7 +477,7 @@ __global__ void __launch_bounds__(256) k_int_vel(Dev D) {
 // Raw values are written first, then divided by the diagonal (Jacobi,
 // isph.f90:432-440) once the diagonal is known.
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_ppe_assemble(Dev D, int want32) {
+__global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
   PIDX();
   if (i >= n) return;
   const DevParams& P = D.P;
@@ -583,9 +583,6 @@ __global__ void __launch_bounds__(256) k_ppe_assemble(Dev D, int want32) {
   for (int k = 0; k < L; k++) {
     if (k == dpos || (quirk && k == dpos + 1)) continue;
     D.aval[k * cap + i] = D.aval[k * cap + i] / t1;
-  }
-  if (want32) {
-    for (int k = 0; k < L; k++) D.aval32[k * cap + i] = (float)D.aval[k * cap + i];
   }
   D.alen[i] = L;
   D.b[i] = rhs / t1;

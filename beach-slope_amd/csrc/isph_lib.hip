@@ -161,9 +161,9 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   per += 4 * 3;                                 // bin, rank, scan scratch
   per += 4 + ROW_CAP * (4 + 8) + 2 * 8;         // PPE rows, rhs, solution
   per += 6 * 8;                                 // FP64 solver vectors
-  // precision form: FP32 matrix, packed column offsets, FP32 vectors, level-0
-  // multigrid vectors, setup rows
-  if (precision) per += ROW_CAP * 4 + 4 * ((ROW_CAP + 1) / 2) + 5 * 4 + 2 * 4 + 9 * 8;
+  // precision form: FP16 V-cycle matrix and packed column offsets, FP32
+  // vectors, level-0 multigrid vectors, setup rows
+  if (precision) per += 2 * 4 * ((ROW_CAP + 1) / 2) + 5 * 4 + 2 * 4 + 9 * 8;
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
   b += 8 * 2 * (n_cap / 256 + 2);               // reduction partials
   if (precision) b += 4 * 12 * ncell + 8 * MG_DENSE_MAX * MG_DENSE_MAX;  // cell levels
@@ -232,7 +232,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   rc |= dalloc(c, &D.sc, 1);
   rc |= dalloc(c, &c->d_bn2, 1);
   if (params->precision) {
-    rc |= dalloc(c, &D.aval32, N * ROW_CAP);
+    rc |= dalloc(c, &D.av16, N * ((ROW_CAP + 1) / 2));
     rc |= dalloc(c, &D.aoff, N * ((ROW_CAP + 1) / 2));
     float** ff[] = {&D.r32, &D.rh32, &D.pp32, &D.sv32, &D.d32};
     for (auto p : ff) rc |= dalloc(c, p, N);
@@ -334,8 +334,8 @@ extern "C" const char* isph_last_error(const isph_ctx* c) {
 extern "C" int isph_set_params(isph_ctx* c, const isph_params* p) {
   if (!c || !p) return ISPH_EARG;
   if (p->cellx != c->P.cellx || p->celly != c->P.celly) return ISPH_EARG;
-  if (p->precision && !c->D.aval32) {
-    set_err(c, "context was created without the FP32 buffers");
+  if (p->precision && !c->D.av16) {
+    set_err(c, "context was created without the precision-form buffers");
     return ISPH_EARG;
   }
   c->P = *p;
@@ -766,7 +766,7 @@ __global__ void k_zero_nnz(Dev D) {
 
 static int assemble_nocheck(isph_ctx* c) {
   hipLaunchKernelGGL(k_zero_nnz, dim3(1), dim3(64), 0, c->stream, c->D);
-  LAUNCH(k_ppe_assemble, c->n_hi, c->D, c->P.precision ? 1 : 0);
+  LAUNCH(k_ppe_assemble, c->n_hi, c->D);
   c->assembled = true;
   return 0;
 }
@@ -1602,7 +1602,6 @@ extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, cons
   if (!c || n < 0 || n > c->cap) return ISPH_EARG;
   std::vector<int32_t> L(n), C((size_t)ROW_CAP * c->cap, 0);
   std::vector<double> V((size_t)ROW_CAP * c->cap, 0.0);
-  std::vector<float> V32((size_t)ROW_CAP * c->cap, 0.0f);
   for (int i = 0; i < n; i++) {
     if (len[i] < 0 || len[i] > ROW_CAP) return ISPH_EARG;
     L[i] = len[i];
@@ -1611,14 +1610,11 @@ extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, cons
       if (j < 0 || j >= n) return ISPH_EARG;
       C[(size_t)k * c->cap + i] = j;
       V[(size_t)k * c->cap + i] = val[(size_t)i * ROW_CAP + k];
-      V32[(size_t)k * c->cap + i] = (float)val[(size_t)i * ROW_CAP + k];
     }
   }
   HIPCHK(c, hipMemcpy(c->D.alen, L.data(), 4 * (size_t)n, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->D.acol, C.data(), 4 * C.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->D.aval, V.data(), 8 * V.size(), hipMemcpyHostToDevice));
-  if (c->D.aval32)
-    HIPCHK(c, hipMemcpy(c->D.aval32, V32.data(), 4 * V32.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->D.b, rhs, 8 * (size_t)n, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->D.sol, x0, 8 * (size_t)n, hipMemcpyHostToDevice));
   DevScalars s;

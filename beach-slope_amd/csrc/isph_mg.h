@@ -115,8 +115,17 @@ __device__ __forceinline__ int mg_nbr(const MgLevel& L, int I, int k) {
 // ---------------------------------------------------------------------
 // setup (once per solve)
 // ---------------------------------------------------------------------
+// FP16 bits of v, saturated at the largest finite half (a Jacobi-scaled row
+// stays near 1; the saturation only keeps a degenerate row finite)
+__device__ __forceinline__ unsigned short mg_half_bits(float v) {
+  const float m = 65504.0f;
+  const float cl = v > m ? m : (v < -m ? -m : v);
+  return __half_as_ushort(__float2half_rn(cl));
+}
+
 // level 0, row i: inverse row diagonal (diagonal plus the slot-1 value of
-// ghost rows), and the row's packed 16-bit column offsets (a row with an
+// ghost rows), the row's FP16 values for the V-cycle, and its packed 16-bit
+// column offsets (a row with an
 // offset beyond 16 bits raises the scenario's far flag, and the solve reads
 // the int32 columns)
 __device__ __forceinline__ void mg_setup0_row(const Dev& D, const MgDev& G, int i) {
@@ -133,6 +142,9 @@ __device__ __forceinline__ void mg_setup0_row(const Dev& D, const MgDev& G, int 
     far = far || o0 < -32768 || o0 > 32767 || o1 < -32768 || o1 > 32767;
     D.aoff[(k >> 1) * cap + i] = (uint32_t)(uint16_t)(int16_t)o0 |
                                  ((uint32_t)(uint16_t)(int16_t)o1 << 16);
+    const float v0 = (float)D.aval[k * cap + i];
+    const float v1 = (k + 1 < L) ? (float)D.aval[(k + 1) * cap + i] : 0.0f;
+    D.av16[(k >> 1) * cap + i] = (uint32_t)mg_half_bits(v0) | ((uint32_t)mg_half_bits(v1) << 16);
   }
   if (far) atomicOr(&D.sc->far, 1);
   G.d0[i] = (d != 0.0 && isfinite(d)) ? (float)(1.0 / d) : 1.0f;
@@ -289,7 +301,7 @@ __device__ __forceinline__ void mg_pre0_row(const Dev& D, const MgDev& G,
                                             int i) {
   if (*G.done || i >= D.sc->n) return;
   const float* __restrict__ x0 = G.x0;
-  const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
+  const float acc = ell_row_sum_h<OFF, ELL_UP>(D, i,
                                               [&](int j) { return x0[j]; });
   r0[i] = (float)v[i] - acc;
 }
@@ -605,7 +617,7 @@ __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
                                              int i) {
   if (*G.done || i >= D.sc->n) return;
   const float* __restrict__ x1 = G.x0;
-  const float acc = ell_row_sum_any<OFF, ELL_UP, float>(D, D.aval32, i,
+  const float acc = ell_row_sum_h<OFF, ELL_UP>(D, i,
                                               [&](int j) { return x1[j]; });
   out[i] = x1[i] + G.w * ((float)v[i] - acc) * G.d0[i];
 }
