@@ -718,8 +718,11 @@ __device__ __forceinline__ void pb_4_rows(const Dev& D, const MgDev& G, int vb) 
 
 // r = b - A64 x (FP64 true residual), and a fresh start of the Krylov space
 // from it: rh32 = p32 = (float) r; partials (r . rh), (r . r).
-// init = 1: start of the solve; a scenario whose right-hand side is zero
-//           (finished by the norm) gets x = 0 instead.
+// init = 1: start of the solve from x = 0, so r = b.  The previous solution
+//           by row index (the reference's fsol, memory.f90:114-144) is a
+//           poor start here: on the 650k flume mid-run its residual is
+//           1.8e3 ||b|| and the solve needs 47 iterations against 38 from 0
+//           (the particles' own previous pressures: 2.2e3 ||b||, 45).
 // init = 2: check of a scenario whose iteration stopped (done, not final);
 //           if the check fails the iteration restarts from this state.
 __device__ __forceinline__ void pb_resid_rows(const Dev& D, const MgDev& G, int vb, int init,
@@ -728,16 +731,21 @@ __device__ __forceinline__ void pb_resid_rows(const Dev& D, const MgDev& G, int 
   const int i = vb * RED_BLOCK + threadIdx.x;
   const int n = S->n;
   if (init == 1 && S->fin) {
-    if (S->bnorm == 0.0 && i < n) D.sol[i] = 0.0;
+    if (i < n) D.sol[i] = 0.0;
     return;
   }
   if (init == 2 && !(S->done && !S->fin)) return;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
-    const double* __restrict__ sol = D.sol;
-    const double acc = ell_row_sum<ELL_U, double>(D.acol, D.aval, D.cap, i, D.alen[i],
-                                                  [&](int j) { return sol[j]; });
-    const double r = D.b[i] - acc;
+    double r;
+    if (init == 1) {
+      D.sol[i] = 0.0;
+      r = D.b[i];
+    } else {
+      const double* __restrict__ sol = D.sol;
+      r = D.b[i] - ell_row_sum<ELL_U, double>(D.acol, D.aval, D.cap, i, D.alen[i],
+                                              [&](int j) { return sol[j]; });
+    }
     D.r[i] = r;
     D.rh32[i] = (float)r;
     D.pp32[i] = (float)r;

@@ -241,7 +241,9 @@ int isph_dump_props(isph_ctx *ctx, double *porosity, double *resist,
 int isph_dump_ppe(isph_ctx *ctx, int32_t *row_len, int32_t *col, double *val,
                   double *rhs);
 /* Upload an external PPE right-hand side / initial guess and solve only
- * (for solver tests on the assembled matrix). */
+ * (for solver tests on the assembled matrix).  The initial guess is the
+ * FP64 form's (like the reference's fsol); the precision form starts every
+ * solve from x = 0. */
 int isph_ppe_set_rhs(isph_ctx *ctx, const double *rhs, const double *x0);
 int isph_ppe_get_solution(isph_ctx *ctx, double *x);
 /* Load a whole PPE system (n rows in the canonical row-major layout of
