@@ -951,9 +951,9 @@ static int bh_iterate(BatchHost* b, const BatchDev& B, int grow, int chunk) {
 }
 
 // Precision-form solve of every scenario of b: cell-multigrid right-
-// preconditioned BiCGStab with an FP32 V-cycle (matrix copy and vectors) and
-// FP32 search directions; FP64 outer products A ph and A sh, solution,
-// residual and dots.  A recursive-residual convergence claim
+// preconditioned BiCGStab with a V-cycle in FP16 (level-0 matrix) and FP32
+// (vectors) and FP32 search directions; FP64 outer products A ph and A sh,
+// solution, residual and dots; every solve starts from x = 0.  A recursive-residual convergence claim
 // is checked against b - A64 x after the chunk in which it was made; when the
 // check fails the iteration continues from the FP64 residual.  The host only
 // polls whether every scenario has finished; stats[s] gets scenario s's.

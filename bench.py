@@ -10,8 +10,8 @@ device resident before the timed region starts.
 
 Two forms are measured on the same input:
   plain   FP64 BiCGStab (the parity path);
-  chosen  BiCGStab right-preconditioned by a cell-multigrid V-cycle: FP32
-          V-cycle and search directions, FP64 outer products, solution,
+  chosen  BiCGStab right-preconditioned by a cell-multigrid V-cycle: FP16/FP32
+          V-cycle, FP32 search directions, FP64 outer products, solution,
           residual and dots, with a final FP64 residual check.
 Both solve to the same relative residual (1e-8) so their throughputs compare
 like for like.  `value` is the chosen form.

@@ -59,10 +59,10 @@ enum { ISPH_F_BUFFER = 1, ISPH_F_FREE = 2, ISPH_F_VICINITY = 4 };
  *   ISPH_PPE_FP64        BiCGStab in FP64, iterates identical to the CPU
  *                        restatement (the parity path).
  *   ISPH_PPE_FP32_REFINE BiCGStab right-preconditioned by a cell-aggregation
- *                        multigrid V-cycle; FP32 V-cycle (matrix copy and
- *                        vectors) and search directions; FP64 products of the
- *                        matrix with the directions, solution, residual and
- *                        dot products;
+ *                        multigrid V-cycle (FP16 level-0 matrix, FP32
+ *                        vectors); FP32 search directions; FP64 products of
+ *                        the matrix with the directions, solution, residual
+ *                        and dot products; every solve starts from x = 0;
  *                        a convergence claim is checked against the FP64
  *                        residual b - A x and the iteration restarts from it
  *                        when the check fails (iterative refinement). */

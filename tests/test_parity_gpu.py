@@ -208,8 +208,8 @@ def test_fp64_breakdown_restart_bit_exact(pkg, oracle, cases):
 
 
 def test_precision_form_residual_and_pressure_error(pkg, oracle, cases):
-    """Precision form (cell-multigrid preconditioned BiCGStab, FP32 V-cycle
-    and directions, FP64 outer products / solution / residual / dots): every solve
+    """Precision form (cell-multigrid preconditioned BiCGStab, FP16/FP32 V-cycle,
+    FP32 directions, FP64 outer products / solution / residual / dots): every solve
     meets the FP64 relative residual target, and on the same assembled system
     the pressure agrees with a tightly converged FP64 solve."""
     c = cases.dam_break(jitter=0.01)
