@@ -354,15 +354,30 @@ __global__ void k_eddyvis(Dev D) {
   const double xi = D.xnb[i], yi = D.ynb[i];
   const double ui = D.u[i] / D.por[i], vi = D.v[i] / D.por[i];
   double t1 = 0.0, t2 = 0.0, t3 = 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = D.nbr[k * cap + i];
-    const Klt z = d_klt(xi - D.xnb[j], yi - D.ynb[j], P);
-    const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
-    const double nj = D.por[j], mj = D.m[j], rj = D.rho[j];
-    const double du = (D.u[j] / nj) - ui, dv = (D.v[j] / nj) - vi;
-    t1 = t1 + ((mj * du) * gx) / rj;
-    t2 = t2 + ((mj * dv) * gy) / rj;
-    t3 = t3 + ((0.50 * mj) * ((du * gy) + (dv * gx))) / rj;
+  // neighbours two at a time: both neighbours' fields are requested before
+  // either is used (the sums stay in list order)
+  for (int k0 = 0; k0 < cnt; k0 += 2) {
+    int jb[2];
+    double xb[2], yb[2], nb[2], mb[2], rb[2], ub[2], vb[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) jb[q] = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int j = jb[q];
+      xb[q] = D.xnb[j]; yb[q] = D.ynb[j]; nb[q] = D.por[j]; mb[q] = D.m[j];
+      rb[q] = D.rho[j]; ub[q] = D.u[j]; vb[q] = D.v[j];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (k0 + q >= cnt) break;
+      const Klt z = d_klt(xi - xb[q], yi - yb[q], P);
+      const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+      const double nj = nb[q], mj = mb[q], rj = rb[q];
+      const double du = (ub[q] / nj) - ui, dv = (vb[q] / nj) - vi;
+      t1 = t1 + ((mj * du) * gx) / rj;
+      t2 = t2 + ((mj * dv) * gy) / rj;
+      t3 = t3 + ((0.50 * mj) * ((du * gy) + (dv * gx))) / rj;
+    }
   }
   const double t4 = sqrt((2 * ((t1 * t1) + (t2 * t2))) + ((2 * t3) * (2 * t3)));
   const double dl = P.dl;
@@ -608,14 +623,29 @@ __global__ void __launch_bounds__(256) k_comp_vel(Dev D) {
     const double xnbi = D.xnb[i], ynbi = D.ynb[i];
     const double ni = D.por[i], rhoi = D.rho[i], pi = D.p[i];
     double u = 0.0, v = 0.0;
-    for (int k = 0; k < cnt; k++) {
-      const int j = D.nbr[k * cap + i];
-      const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
+    // neighbours two at a time: both neighbours' fields are requested before
+    // either is used (the sums stay in list order)
+    for (int k0 = 0; k0 < cnt; k0 += 2) {
+      int jb[2];
+      double xb[2], yb[2], nb[2], rb[2], pb[2], mb[2];
+      int tb[2];
+#pragma unroll
+      for (int q = 0; q < 2; q++) jb[q] = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const int j = jb[q];
+        xb[q] = D.xnb[j]; yb[q] = D.ynb[j]; nb[q] = D.por[j]; rb[q] = D.rho[j];
+        pb[q] = D.p[j]; mb[q] = D.m[j]; tb[q] = D.tid[j];
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+      if (k0 + q >= cnt) break;
+      const Klt z = d_klt(xnbi - xb[q], ynbi - yb[q], P);
       const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
-      const double nj = D.por[j], rhoj = D.rho[j], pj = D.p[j];
-      const double V = D.m[j] / rhoj;
+      const double nj = nb[q], rhoj = rb[q], pj = pb[q];
+      const double V = mb[q] / rhoj;
       double t1;
-      if (D.tid[j] == TID_FLUID) {
+      if (tb[q] == TID_FLUID) {
         const double a = ((pj * rhoi) / ni) + ((pi * rhoj) / nj);
         const double den = rhoi * ((rhoi / ni) + (rhoj / nj));
         t1 = (2 * (((V * (ni * ni)) * a) * gx)) / den;
@@ -627,6 +657,7 @@ __global__ void __launch_bounds__(256) k_comp_vel(Dev D) {
         u = u - t1 * dt;
         t1 = (((V * (ni * ni)) * (pj + pi)) * gy) / rhoi;
         v = v - t1 * dt;
+      }
       }
     }
     u = D.us[i] + u;
@@ -688,18 +719,33 @@ __global__ void __launch_bounds__(256) k_opt2_shift(Dev D) {
   const double coef = (P.csh * (P.h * P.h)) * frac;
   const double ui = D.u[i], vi = D.v[i];
   double xsa = 0.0, ysa = 0.0, a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = D.nbr[k * cap + i];
-    const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
-    const double V = D.m[j] / D.rho[j];
-    xsa = xsa - ((coef * z.k1) * V);
-    ysa = ysa - ((coef * z.k2) * V);
-    const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
-    const double du = D.u[j] - ui, dv = D.v[j] - vi;
-    a1 = a1 + (gx * du) * V;
-    a2 = a2 + (gy * du) * V;
-    b1 = b1 + (gx * dv) * V;
-    b2 = b2 + (gy * dv) * V;
+  // neighbours two at a time: both neighbours' fields are requested before
+  // either is used (the sums stay in list order)
+  for (int k0 = 0; k0 < cnt; k0 += 2) {
+    int jb[2];
+    double xb[2], yb[2], mb[2], rb[2], ub[2], vb[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) jb[q] = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int j = jb[q];
+      xb[q] = D.xnb[j]; yb[q] = D.ynb[j]; mb[q] = D.m[j]; rb[q] = D.rho[j];
+      ub[q] = D.u[j]; vb[q] = D.v[j];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (k0 + q >= cnt) break;
+      const Klt z = d_klt(xnbi - xb[q], ynbi - yb[q], P);
+      const double V = mb[q] / rb[q];
+      xsa = xsa - ((coef * z.k1) * V);
+      ysa = ysa - ((coef * z.k2) * V);
+      const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+      const double du = ub[q] - ui, dv = vb[q] - vi;
+      a1 = a1 + (gx * du) * V;
+      a2 = a2 + (gy * du) * V;
+      b1 = b1 + (gx * dv) * V;
+      b2 = b2 + (gy * dv) * V;
+    }
   }
   const double capv = P.maxshift * P.dl;
   const double ax = fabs(xsa), ay = fabs(ysa);
