@@ -430,11 +430,28 @@ __global__ void __launch_bounds__(256) k_int_vel(Dev D) {
     const double nuti = D.nut[i], tkei = D.tke[i];
     const double cmu_wall = visi * (1.0 / r0i);
     double us = 0.0, vs = 0.0;
-    for (int k = 0; k < cnt; k++) {
-      const int j = D.nbr[k * cap + i];
-      const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
+    // neighbours two at a time: both neighbours' unconditional fields are
+    // requested before either is used (the sums stay in list order)
+    for (int k0 = 0; k0 < cnt; k0 += 2) {
+    int jb[2], tb[2];
+    uint8_t fb[2];
+    double xnb_[2], ynb_[2], mb[2], rb[2], xb[2], yb[2], ub[2], vb[2], kb[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) jb[q] = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int j = jb[q];
+      xnb_[q] = D.xnb[j]; ynb_[q] = D.ynb[j]; tb[q] = D.tid[j]; fb[q] = D.flags[j];
+      mb[q] = D.m[j]; rb[q] = D.rho[j]; xb[q] = D.x[j]; yb[q] = D.y[j];
+      ub[q] = D.u[j]; vb[q] = D.v[j]; kb[q] = D.tke[j];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (k0 + q >= cnt) break;
+      const int j = jb[q];
+      const Klt z = d_klt(xnbi - xnb_[q], ynbi - ynb_[q], P);
       const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
-      const int tj = D.tid[j];
+      const int tj = tb[q];
       double cmu;
       if (tj == TID_FLUID) {
         const double r0j = D.rho0[j], visj = d_vis(r0j);
@@ -444,12 +461,12 @@ __global__ void __launch_bounds__(256) k_int_vel(Dev D) {
       }
       double vart;
       if (freei) vart = vartsi;
-      else if ((D.flags[j] & F_FREE) && !buf) vart = (d_wab(z.r, P) * D.varts[j]) / P.wo;
+      else if ((fb[q] & F_FREE) && !buf) vart = (d_wab(z.r, P) * D.varts[j]) / P.wo;
       else vart = 0.0;
-      const double pvol = D.m[j] / D.rho[j];
+      const double pvol = mb[q] / rb[q];
       const double pd = 1.0 / ((z.r * z.r) + P.lam);
-      const double dxij = xi - D.x[j], dyij = yi - D.y[j];
-      const double uj = D.u[j], vj = D.v[j];
+      const double dxij = xi - xb[q], dyij = yi - yb[q];
+      const double uj = ub[q], vj = vb[q];
       double a1 = ((((2 * pvol) * (ui - uj)) * dxij) * gx) * pd;
       double a2 = ((((2 * pvol) * (ui - uj)) * dyij) * gy) * pd;
       us = us + (a1 + a2) * (cmu + vart);
@@ -466,11 +483,12 @@ __global__ void __launch_bounds__(256) k_int_vel(Dev D) {
         if ((nuti < eps6) && (nutj < eps6)) vs = vs + ((a1 + a2) * (nuti + nutj)) * 0.50;
         else vs = vs + (a1 + a2) * (((2 * nuti) * nutj) / (nuti + nutj));
       }
-      const double dk = pvol * (D.tke[j] - tkei);
+      const double dk = pvol * (kb[q] - tkei);
       double tk = dk * gx;
       us = us - ((tk * ni) * 2.0) / 3;
       tk = dk * gy;
       vs = vs - ((tk * ni) * 2.0) / 3;
+    }
     }
     const double den = 1.0 - ((dt * D.resist[i]) * ni);
     us = (ui + (dt * us)) / den;
