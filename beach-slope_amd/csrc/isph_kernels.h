@@ -527,23 +527,40 @@ __global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
   double diag = 0.0, rhs = 0.0;
   int L = 0, dpos = 0;
   bool before = true, first_written = false;
-  for (int k = 0; k < cnt; k++) {
-    const int j = D.nbr[k * cap + i];
+  // neighbours two at a time: both neighbours' common fields are requested
+  // before either is used (entries and sums stay in list order)
+  for (int k0 = 0; k0 < cnt; k0 += 2) {
+  int jb[2], tb[2];
+  double nb_[2], rb_[2], mb_[2], xb_[2], yb_[2], xnb_[2], ynb_[2], usb[2], vsb[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) jb[q] = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int j = jb[q];
+    tb[q] = D.tid[j]; nb_[q] = D.por[j]; rb_[q] = D.rho[j]; mb_[q] = D.m[j];
+    xb_[q] = D.x[j]; yb_[q] = D.y[j]; xnb_[q] = D.xnb[j]; ynb_[q] = D.ynb[j];
+    usb[q] = D.us[j]; vsb[q] = D.vs[j];
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int k = k0 + q;
+    if (k >= cnt) break;
+    const int j = jb[q];
     if (before && j > i) {
       // reserve the diagonal slot (and the ghost-row slot-1 value)
       before = false;
       dpos = L;
       L += 1 + ((tidi == TID_GHOST && k > 0 && !first_written) ? 1 : 0);
     }
-    const int tj = D.tid[j];
-    const double nj = D.por[j], rhoj = D.rho[j];
+    const int tj = tb[q];
+    const double nj = nb_[q], rhoj = rb_[q];
     const double rho_j = rhoj * (1.0 / nj);
-    const double pvol = D.m[j] * (1.0 / rhoj);
-    const double xj = D.x[j], yj = D.y[j];
+    const double pvol = mb_[q] * (1.0 / rhoj);
+    const double xj = xb_[q], yj = yb_[q];
     const double ddx = xj - xi, ddy = yj - yi;
     const double dd = sqrt((ddx * ddx) + (ddy * ddy));
     const double pd = 1.0 / ((dd * dd) + P.lam);
-    const Klt z = d_klt(xnbi - D.xnb[j], ynbi - D.ynb[j], P);
+    const Klt z = d_klt(xnbi - xnb_[q], ynbi - ynb_[q], P);
     const double dx = xi - xj, dy = yi - yj;
     bool wr = false;
     double val = 0.0, t1, t2;
@@ -558,11 +575,11 @@ __global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
       wr = true;
       diag = diag + (t1 + t2) * lamp;
       if (tj == TID_FLUID) {
-        t1 = (pvol * (((D.us[j] * rho_ij) / nj) - ((usi * rho_ij) / ni))) * gx;
-        t2 = (pvol * (((D.vs[j] * rho_ij) / nj) - ((vsi * rho_ij) / ni))) * gy;
+        t1 = (pvol * (((usb[q] * rho_ij) / nj) - ((usi * rho_ij) / ni))) * gx;
+        t2 = (pvol * (((vsb[q] * rho_ij) / nj) - ((vsi * rho_ij) / ni))) * gy;
       } else {
-        t1 = (pvol * (((D.us[j] * rho_i) / ni) - ((usi * rho_i) / ni))) * gx;
-        t2 = (pvol * (((D.vs[j] * rho_i) / ni) - ((vsi * rho_i) / ni))) * gy;
+        t1 = (pvol * (((usb[q] * rho_i) / ni) - ((usi * rho_i) / ni))) * gx;
+        t2 = (pvol * (((vsb[q] * rho_i) / ni) - ((vsi * rho_i) / ni))) * gy;
       }
       rhs = rhs + (t1 + t2) / dt;
     } else if (tidi == TID_GHOST) {
@@ -599,6 +616,7 @@ __global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
       D.aval[L * cap + i] = val;
       L++;
     }
+  }
   }
   if (before) {
     dpos = L;
