@@ -873,13 +873,25 @@ __global__ void __launch_bounds__(256) k_scalart(Dev D) {
   const double nuti = D.nut[i], vol0i = D.vol0[i];
   const double vi = sqrt((ui * ui) + (vi_ * vi_)) / ni;
   double cd = 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = D.nbr[k * cap + i];
-    if (D.tid[j] != TID_FLUID) continue;
-    const double ex = xnbi - D.xnb[j], ey = ynbi - D.ynb[j];
+  // neighbours two at a time: both neighbours' fields are requested before
+  // either is used (the sum stays in list order)
+  for (int k0 = 0; k0 < cnt; k0 += 2) {
+  int tb[2];
+  double xnb_[2], ynb_[2], ub[2], vb[2], nb[2], cb[2], xb[2], yb[2], nutb[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int j = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+    tb[q] = D.tid[j]; xnb_[q] = D.xnb[j]; ynb_[q] = D.ynb[j]; ub[q] = D.u[j]; vb[q] = D.v[j];
+    nb[q] = D.por[j]; cb[q] = D.c[j]; xb[q] = D.x[j]; yb[q] = D.y[j]; nutb[q] = D.nut[j];
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (k0 + q >= cnt) break;
+    if (tb[q] != TID_FLUID) continue;
+    const double ex = xnbi - xnb_[q], ey = ynbi - ynb_[q];
     const double r = sqrt((ex * ex) + (ey * ey));
-    const double uj = D.u[j], vj_ = D.v[j], nj = D.por[j], c2 = D.c[j];
-    const double dx = xi - D.x[j], dy = yi - D.y[j];
+    const double uj = ub[q], vj_ = vb[q], nj = nb[q], c2 = cb[q];
+    const double dx = xi - xb[q], dy = yi - yb[q];
     double t1x;
     if ((ni < thr) && (nj < thr)) {
       const double vj = sqrt((uj * uj) + (vj_ * vj_)) / nj;
@@ -898,7 +910,7 @@ __global__ void __launch_bounds__(256) k_scalart(Dev D) {
       t1x = 1.0 / t1x;
       t1x = (t1x <= capv) ? t1x : capv;
     } else {
-      const double nutj = D.nut[j];
+      const double nutj = nutb[q];
       const double Dxxi = Dm + nuti / P.tschmidt;
       const double Dxxj = Dm + nutj / P.tschmidt;
       if ((nuti < eps6) && (nutj < eps6)) t1x = (Dxxi + Dxxj) * 0.50;
@@ -909,6 +921,7 @@ __global__ void __launch_bounds__(256) k_scalart(Dev D) {
     const double den = (r * r) + P.lam;
     cd = (cd + (((((((t1x * 1.0) * t3) * (ci1 - c2)) * dx) * hwx) / den))) +
          (((((((t1x * 1.0) * t3) * (ci1 - c2)) * dy) * hwy) / den));
+  }
   }
   D.cdiff[i] = cd;
 }
