@@ -228,8 +228,9 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
   // Two passes, so the FP64 work of the accepted pairs runs with the lanes of
   // a wavefront together rather than under the candidate test's divergence:
   // the candidates are tested and the list written first, then the list is
-  // walked in its order.  r is recomputed from the same operands, so every
-  // value and sum is the one of a single pass.
+  // walked in its order, two neighbours' fields requested before either is
+  // used.  r is recomputed from the same operands, so every value and sum is
+  // the one of a single pass.
 #pragma unroll 1
   for (int half = 0; half < 2; half++) {
     const int k0 = (c0 + half) * P.celly + r0;
@@ -252,19 +253,28 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
     cnt = NBR_CAP;
   }
   double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
-  for (int mm = 0; mm < cnt; mm++) {
-    const int j = D.nbr[mm * cap + i];
-    const double xj = D.x[j], yj = D.y[j];
-    const double dx = xi - xj, dy = yi - yj;
-    const double r = sqrt((dx * dx) + (dy * dy));
-    const double k1 = d_wabd(dx, r, P), k2 = d_wabd(dy, r, P);
-    const double mj = D.m[j], rj = D.rho[j];
-    const double r1 = (mj * (xj - xi)) / rj;
-    const double r3 = (mj * (yj - yi)) / rj;
-    m0 = m0 + k1 * r1;
-    m1 = m1 + k2 * r1;
-    m2 = m2 + k1 * r3;
-    m3 = m3 + k2 * r3;
+  for (int m0k = 0; m0k < cnt; m0k += 2) {
+    double xb[2], yb[2], mb[2], rb[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int j = (m0k + q < cnt) ? D.nbr[(m0k + q) * cap + i] : i;
+      xb[q] = D.x[j]; yb[q] = D.y[j]; mb[q] = D.m[j]; rb[q] = D.rho[j];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (m0k + q >= cnt) break;
+      const double xj = xb[q], yj = yb[q];
+      const double dx = xi - xj, dy = yi - yj;
+      const double r = sqrt((dx * dx) + (dy * dy));
+      const double k1 = d_wabd(dx, r, P), k2 = d_wabd(dy, r, P);
+      const double mj = mb[q], rj = rb[q];
+      const double r1 = (mj * (xj - xi)) / rj;
+      const double r3 = (mj * (yj - yi)) / rj;
+      m0 = m0 + k1 * r1;
+      m1 = m1 + k2 * r1;
+      m2 = m2 + k1 * r3;
+      m3 = m3 + k2 * r3;
+    }
   }
   D.ncnt[i] = cnt;
   D.xnb[i] = xi;
