@@ -27,7 +27,10 @@ of the same workload.
 Three more legs run by default, at bounded sizes, and go into the same line:
   precision_errors  the jittered dam break, 200 steps in each form: pressure,
            free-surface, front and runup differences of the precision form
-           against the FP64 run (gauges.py);
+           against the FP64 run (gauges.py); and, since the runup does not
+           develop in those 200 steps, the same dam break at dx 0.01 m with
+           the toe 5 cm beyond the column for 1,250 steps (runup, front and
+           slope-surface differences);
   batched  64 independent flumes, N ~ U[20k, 300k] (about 10M particles),
            precision form, stepped together (one set of solver launches for
            all), against the same scenarios stepped one after another alone;
@@ -208,6 +211,46 @@ def run_precision_errors(pkg, cases, steps: int) -> dict:
     return out
 
 
+def run_runup_errors(pkg, cases, steps: int) -> dict:
+    """Runup against the FP64 run where it develops: the C3 dam break has its
+    toe 0.5 m beyond the column, out of reach in its 200 steps, so the same
+    dam break at dx = 0.01 m with the toe 5 cm beyond the column and seeded
+    +-0.01 dx jitter is run `steps` steps in both forms (the front reaches
+    the slope by step ~700).  A run that stops on an isolated particle (a
+    zero PPE diagonal, as the reference's assembly would produce NaN rows)
+    is reported instead of compared."""
+    from beach_slope_amd import gauges
+    c = cases.dam_break(dx=0.01, jitter=0.01, toe=0.55)
+    m = c.meta
+    out = {"workload": f"dam_break_jitter dx=0.01 toe=0.55 n={c.n}", "steps": steps}
+    states = {}
+    for form, prec, tol in (("fp64", 0, 1e-10), ("precision", 1, 1e-8)):
+        prm = dict(c.params, precision=prec, tol=tol, max_outer=20)
+        sc = pkg.Scenario(c, params=prm)
+        sc.init_state()
+        t0 = time.perf_counter()
+        try:
+            for k in range(steps):
+                sc.step(1)
+        except pkg.IsphError as e:
+            out[f"{form}_stopped"] = f"step {k}: {e}"
+            sc.close()
+            return out
+        out[f"{form}_seconds"] = time.perf_counter() - t0
+        states[form] = sc.download()
+        sc.close()
+    a, b = states["precision"], states["fp64"]
+    stations = m["toe"] + np.array([0.02, 0.05, 0.10, 0.15])
+    ea = gauges.surface_elevation(a, stations, 0.5 * m["dx"])
+    eb = gauges.surface_elevation(b, stations, 0.5 * m["dx"])
+    ra, rb = gauges.runup(a, m["toe"], m["off"]), gauges.runup(b, m["toe"], m["off"])
+    out.update(runup_m=rb, runup_abs_m=abs(ra - rb), runup_rel=abs(ra - rb) / rb if rb else None,
+               front_m=gauges.front_position(b),
+               front_abs_m=abs(gauges.front_position(a) - gauges.front_position(b)),
+               slope_surface_max_abs_m=float(np.nanmax(np.abs(ea - eb))))
+    return out
+
+
 KERNEL_OF = {"spmv_s": "k_bicg_2", "spmv_s32": "kb_pb_2"}
 TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_flume_650k.md"
 
@@ -264,6 +307,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--error-steps", type=int, default=200)
+    ap.add_argument("--runup-steps", type=int, default=1250)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-scenarios", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=2)
@@ -294,6 +338,8 @@ def main() -> int:
     batched = served = errors = None
     if not args.no_errors:
         errors = run_precision_errors(pkg, cases, args.error_steps)
+        if args.runup_steps > 0:
+            errors["runup"] = run_runup_errors(pkg, cases, args.runup_steps)
         print(f"bench: precision errors {errors}", file=sys.stderr, flush=True)
     if not args.no_batch:
         batched = run_batch(pkg, cases, args.batch_scenarios, args.batch_steps, 1)
