@@ -3,7 +3,8 @@
 // and per-scenario device bodies.  The kernels that run them, one launch per
 // phase for any number of scenarios, are in isph_batch.h.
 //
-// Level 0 is the particle matrix itself (FP32 copy of the canonical rows).
+// Level 0 is the particle matrix itself (the canonical rows' values in FP16,
+// converted at setup, summed in FP32).
 // Level 1 aggregates every cell's particles into one unknown: its operator is
 // the Galerkin sum A1(I,J) = sum_{i in I, j in J} A_ij, a 9-point stencil on
 // the regular cell grid (neighbours lie within 2h < 4h, so only adjacent
