@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) kb_mg_coarsen(BatchDev B, int l) {
   BATCH_CELLS(B, BK_COARSEN + l - 1);
   mg_coarsen_cell(G, l, I_);
 }
-__global__ void __launch_bounds__(256) kb_mg_dense(BatchDev B) {
+__global__ void __launch_bounds__(MG_CO_THREADS) kb_mg_dense(BatchDev B) {
   mg_dense_block(B.mgs[blockIdx.x]);
 }
 

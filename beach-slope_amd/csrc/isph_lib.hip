@@ -971,7 +971,7 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   BLAUNCH(kb_mg_stencil1, tot(BK_CELL1), 256, B);
   for (int l = 1; l < b->max_nlev; l++)
     if (tot(BK_COARSEN + l - 1) > 0) BLAUNCH(kb_mg_coarsen, tot(BK_COARSEN + l - 1), 256, B, l);
-  BLAUNCH(kb_mg_dense, S, 256, B);
+  BLAUNCH(kb_mg_dense, S, MG_CO_THREADS, B);
   // ||b||, then the FP64 start residual
   BLAUNCH(kb_pb_bnorm, grow, 256, B);
   BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BNORM, 0);

@@ -29,6 +29,8 @@
 #define MG_MAX_LEVELS 24
 #define MG_DENSE_MAX 64
 #define MG_CO_THREADS 1024
+static_assert(MG_CO_THREADS % (2 * MG_DENSE_MAX) == 0,
+              "the dense inverse's update maps whole rows of 2 MG_DENSE_MAX columns");
 // The first level of the coarse workgroup has at most MG_CO_PER unknowns per
 // thread: their stencils stay in registers for the whole cycle.
 #define MG_CO_PER 3
@@ -277,9 +279,13 @@ __device__ __forceinline__ void mg_dense_block(const MgDev& G) {
     for (int e = t; e < w2; e += T) prow[e] = (pv != 0.0) ? A[c * w2 + e] / pv : 0.0;
     for (int r = t; r < n; r += T) colc[r] = A[r * w2 + c];
     __syncthreads();
-    for (int e = t; e < n * w2; e += T) {
-      const int r = e / w2, q = e - r * w2;
-      A[e] = (r == c) ? prow[q] : A[e] - colc[r] * prow[q];
+    // thread t updates column t mod 2 MG_DENSE_MAX of rows t / (2 MG_DENSE_MAX),
+    // + T / (2 MG_DENSE_MAX), ... (no division in the loop)
+    {
+      const int q = t & (2 * MG_DENSE_MAX - 1), rs = T / (2 * MG_DENSE_MAX);
+      if (q < w2)
+        for (int r = t / (2 * MG_DENSE_MAX); r < n; r += rs)
+          A[r * w2 + q] = (r == c) ? prow[q] : A[r * w2 + q] - colc[r] * prow[q];
     }
     __syncthreads();
   }
