@@ -84,7 +84,8 @@ EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_dump_ppe", "isph_ppe_set_rhs", "isph_ppe_get_solution",
            "isph_load_system", "isph_batch_create", "isph_batch_step", "isph_batch_stats",
            "isph_batch_destroy", "isph_set_buffer", "isph_buffer_update", "isph_set_wcsph",
-           "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read") + tuple("isph_" + p for p in PHASES)
+           "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read",
+           "isph_wavemaker_motion", "isph_set_wavemaker", "isph_wavemaker") + tuple("isph_" + p for p in PHASES)
 
 _lib = None
 
@@ -133,6 +134,9 @@ def lib():
     L.isph_set_buffer.argtypes = [vp, i32, i64] + [vp] * 5 + [d] * 7
     L.isph_buffer_update.argtypes = [vp, i32]
     L.isph_set_wcsph.argtypes = [vp, d, d, d, d, i32]
+    L.isph_wavemaker_motion.argtypes = [d, d, d, d, d, C.POINTER(d), C.POINTER(d)]
+    L.isph_set_wavemaker.argtypes = [vp, d, d, d, d]
+    L.isph_wavemaker.argtypes = [vp]
     L.isph_prof_enable.argtypes = [vp, i32]
     L.isph_prof_read.argtypes = [vp, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
     L.isph_footprint_bytes.restype = i64
@@ -173,6 +177,9 @@ class Scenario:
             self._check(L.isph_upload_solids(self._ctx, case.ns, _ptr(s["x"]), _ptr(s["y"]),
                                              _ptr(s["m"]), _ptr(s["rho"])),
                         "isph_upload_solids")
+        wave = case.meta.get("wavemaker")
+        if wave:
+            self.set_wavemaker(wave["stroke"], wave["period"], wave["ramp"], wave["t0"])
 
     # -- plumbing --------------------------------------------------------
     def _check(self, rc: int, where: str, ctx: bool = True):
@@ -325,6 +332,16 @@ class Scenario:
         self._check(lib().isph_set_wcsph(self._ctx, c0, gamma, delta, swl, int(in_step)),
                     "isph_set_wcsph")
 
+    def set_wavemaker(self, stroke: float, period: float, ramp: float = 0.0, t0: float = 0.0):
+        """Piston wavemaker (isph_set_wavemaker): the ISPH_F_PADDLE walls move by
+        X(t) from their current positions, which are taken as rest + X(now)."""
+        self._check(lib().isph_set_wavemaker(self._ctx, stroke, period, ramp, t0),
+                    "isph_set_wavemaker")
+
+    def wavemaker(self):
+        """Move the paddle to X, U at the context's current time (isph_wavemaker)."""
+        self._check(lib().isph_wavemaker(self._ctx), "isph_wavemaker")
+
     def presstoden(self): self._phase("presstoden")
     def dentopress(self): self._phase("dentopress")
     def calmasscons(self): self._phase("calmasscons")
@@ -399,6 +416,16 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def wavemaker_motion(stroke: float, period: float, ramp: float, t0: float, t: float):
+    """(X, U) of the paddle motion at time t (isph_wavemaker_motion; host code,
+    no device needed)."""
+    x, u = C.c_double(), C.c_double()
+    rc = lib().isph_wavemaker_motion(stroke, period, ramp, t0, t, C.byref(x), C.byref(u))
+    if rc != 0:
+        raise IsphError(rc, "isph_wavemaker_motion")
+    return x.value, u.value
 
 
 def footprint_bytes(n_cap: int, n_solid_cap: int, cellx: int, celly: int,

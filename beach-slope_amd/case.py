@@ -30,7 +30,7 @@ import math
 import numpy as np
 
 TID_FIXED, TID_GHOST, TID_FLUID = 1, 2, 3
-F_BUFFER, F_FREE, F_VICINITY = 1, 2, 4
+F_BUFFER, F_FREE, F_VICINITY, F_PADDLE = 1, 2, 4, 8
 
 RHO = 1000.0
 REL_DEN = 1.025
@@ -97,8 +97,67 @@ def _walls_box(x0: float, x1: float, y0: float, ytop: float, dx: float):
     return np.concatenate(xs), np.concatenate(ys), np.concatenate(ts).astype(np.int32)
 
 
+def wave_number(period: float, depth: float, g: float = abs(G)) -> float:
+    """k of the linear dispersion relation w^2 = g k tanh(k d), by Newton's
+    method (f(k) is increasing and concave, so the iteration from the
+    shallow-water guess w / sqrt(g d), which lies above the root, falls
+    monotonically onto it)."""
+    w = 2.0 * math.pi / period
+    k = w / math.sqrt(g * depth)
+    for _ in range(100):
+        th = math.tanh(k * depth)
+        f = g * k * th - w * w
+        df = g * th + g * k * depth * (1.0 - th * th)
+        dk = f / df
+        k -= dk
+        if abs(dk) <= 1e-16 * k:
+            break
+    return k
+
+
+def piston_stroke(wave_h: float, period: float, depth: float, g: float = abs(G)) -> float:
+    """Piston-wavemaker stroke S for a target wave height H by linear
+    wavemaker (Biesel) theory: H/S = 2 (cosh 2kd - 1) / (sinh 2kd + 2kd)
+    (Dean & Dalrymple, Water Wave Mechanics for Engineers and Scientists,
+    1991, sec. 6.2).  H/S tends to kd in shallow water and to 2 in deep."""
+    kd2 = 2.0 * wave_number(period, depth, g) * depth
+    return wave_h * (math.sinh(kd2) + kd2) / (2.0 * (math.cosh(kd2) - 1.0))
+
+
+def _walls_paddle_flume(x0: float, x1: float, y0: float, ytop: float, dx: float, back: float):
+    """Walls of a flume whose left end is a piston paddle: three bottom layers
+    from x0 - back to the right wall, a three-layer right wall as in
+    `_walls_box`, and a three-layer paddle at x0 - (k + 1/2) dx from y0 + dx/2
+    up (above the bottom, so it slides over it) carrying F_PADDLE.  back
+    covers the paddle's travel behind its rest position.
+    Returns (x, y, tid, flags)."""
+    xs, ys, ts, fs = [], [], [], []
+    nback = int(math.ceil(back / dx))
+    nx = int(round((x1 - x0) / dx))
+    ny = int(round((ytop - y0) / dx))
+    for k in range(3):
+        t = TID_FIXED if k == 0 else TID_GHOST
+        xb = x0 + (np.arange(-nback, nx + k + 1) + 0.5) * dx
+        xs.append(xb)
+        ys.append(np.full(xb.shape, y0 - (k + 0.5) * dx))
+        ts.append(np.full(xb.shape, t))
+        fs.append(np.zeros(xb.shape, np.uint8))
+        yv = y0 + (np.arange(-k, ny) + 0.5) * dx
+        xs.append(np.full(yv.shape, x1 + (k + 0.5) * dx))
+        ys.append(yv)
+        ts.append(np.full(yv.shape, t))
+        fs.append(np.zeros(yv.shape, np.uint8))
+        yp = y0 + (np.arange(0, ny) + 0.5) * dx
+        xs.append(np.full(yp.shape, x0 - (k + 0.5) * dx))
+        ys.append(yp)
+        ts.append(np.full(yp.shape, t))
+        fs.append(np.full(yp.shape, F_PADDLE, np.uint8))
+    return (np.concatenate(xs), np.concatenate(ys), np.concatenate(ts).astype(np.int32),
+            np.concatenate(fs))
+
+
 def _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, depth_top, sx=None, sy=None,
-              steps=1, meta=None) -> Case:
+              steps=1, meta=None, wflags=None) -> Case:
     n_f, n_w = fx.size, wx.size
     x = np.concatenate([wx, fx])
     y = np.concatenate([wy, fy])
@@ -110,9 +169,12 @@ def _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, depth_top, sx=None, sy=N
     vol0 = np.full(x.size, dx * dx)
     m = rho0 * vol0
     p = np.maximum(RHO * abs(G) * (depth_top - y), 0.0)
+    flags = np.zeros(x.size, np.uint8)
+    if wflags is not None:
+        flags[:n_w] = wflags
     parts = dict(
         x=x, y=y, u=np.zeros_like(x), v=np.zeros_like(x), p=p, m=m, rho=rho0.copy(),
-        rho0=rho0, vol0=vol0, c=c_ref, tid=tid, flags=np.zeros(x.size, np.uint8),
+        rho0=rho0, vol0=vol0, c=c_ref, tid=tid, flags=flags,
     )
     if sx is None:
         sx = np.zeros(0)
@@ -169,14 +231,25 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
                  porosity: float = 0.4, d50: float = 0.01,
                  patch=(0.2, 0.1), dt: float = 5.0e-5, steps: int = 2000,
                  wave_h: float = 0.05, wave_t: float = 1.5, grade_h: float = 8.0,
+                 wavemaker: bool = True, ramp: float | None = None,
                  name: str = "porous_flume") -> Case:
     """Wave-runup flume on a porous beach (the 650k-particle configuration at
     the defaults: 4000 x 160 fluid lattice, 3 wall layers).  The 1:10 beach
     under the line y = (x - toe)*slope is filled with static solid particles
     at the fluid spacing; the fluid lattice fills the pores as in the
     reference's seepage case.  A scalar patch of c = 1 (0.2 m x 0.1 m) sits at
-    the toe.  The wavemaker paddle is held at rest: the piston amplitude
-    (wave_h, wave_t) is recorded in meta but not applied.
+    the toe.
+
+    The left end is a piston wavemaker (with `wavemaker`): a three-layer
+    paddle of F_PADDLE walls whose stroke gives waves of height wave_h and
+    period wave_t by linear wavemaker theory (`piston_stroke`), started with a
+    linear ramp over `ramp` seconds (one period by default).  The bottom
+    extends behind the paddle's rest position by half the stroke plus 4 dx,
+    and the flume sits that much further right, rounded up to an even number
+    of cells (meta "off" is the paddle's rest x, "offy" the bottom's y).
+    meta["wavemaker"] holds the motion (stroke, period, ramp, t0) that
+    `Scenario` and the oracle's wrapper apply on upload.  Without it the
+    left end is an ordinary three-layer wall.
 
     The beach surface is graded: solid volume ramps linearly from 0 at the
     beach surface to full at a depth of grade_h*h, so the porosity goes from 1
@@ -186,14 +259,27 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
     steps; a graded layer keeps it bounded."""
     params = default_params(dx)
     params.update(dt=dt, por=porosity, d50=d50, tol=1.0e-10)
-    off = 3.0 * dx + 0.5 * dx
-    wx, wy, wt = _walls_box(off, off + length, off, off + tank_h, dx)
+    wflags = None
+    if wavemaker:
+        stroke = piston_stroke(wave_h, wave_t, depth)
+        back = 0.5 * stroke + 4.0 * dx
+        # the flume moves right by an even number of cells, so the lattice sits
+        # on the cell grid, and cells pair into the first coarse multigrid
+        # level, as without the paddle; it stays at the same height
+        cs = 4.0 * params["h"]
+        off = 3.0 * dx + 0.5 * dx + 2.0 * math.ceil(back / (2.0 * cs)) * cs
+        offy = 3.0 * dx + 0.5 * dx
+        wx, wy, wt, wflags = _walls_paddle_flume(off, off + length, offy, offy + tank_h, dx,
+                                                 back)
+    else:
+        off = offy = 3.0 * dx + 0.5 * dx
+        wx, wy, wt = _walls_box(off, off + length, off, off + tank_h, dx)
     nx, ny = int(round(length / dx)), int(round(depth / dx))
     gx, gy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
     fx = off + (gx.ravel() + 0.5) * dx
-    fy = off + (gy.ravel() + 0.5) * dx
+    fy = offy + (gy.ravel() + 0.5) * dx
     lx = fx - off
-    ly = fy - off
+    ly = fy - offy
     fc = ((np.abs(lx - toe) <= 0.5 * patch[0]) & (ly <= patch[1])).astype(np.float64)
     # porous beach solids
     sxi = np.arange(int(math.floor(toe / dx)), nx)
@@ -205,12 +291,15 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
     weight = np.clip(below[keep] / (grade_h * params["h"]), 0.0, 1.0) if grade_h > 0 \
         else np.ones(int(keep.sum()))
     sx = off + sx[keep]
-    sy = off + sy[keep]
-    _grid(params, off + length + 4 * dx, off + tank_h + 4 * dx)
-    c = _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, off + depth, sx, sy,
-                  steps=steps, meta=dict(dx=dx, length=length, depth=depth, toe=toe,
-                                         slope=slope, porosity=porosity, d50=d50,
-                                         wave_h=wave_h, wave_t=wave_t, grade_h=grade_h))
+    sy = offy + sy[keep]
+    _grid(params, off + length + 4 * dx, offy + tank_h + 4 * dx)
+    meta = dict(dx=dx, length=length, depth=depth, toe=toe, slope=slope, porosity=porosity,
+                d50=d50, wave_h=wave_h, wave_t=wave_t, grade_h=grade_h, off=off, offy=offy)
+    if wavemaker:
+        meta["wavemaker"] = dict(stroke=stroke, period=wave_t,
+                                 ramp=wave_t if ramp is None else ramp, t0=0.0)
+    c = _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, offy + depth, sx, sy,
+                  steps=steps, meta=meta, wflags=wflags)
     c.solids["m"] = c.solids["m"] * weight
     return c
 

@@ -12,6 +12,7 @@
 #define TID_FLUID 3
 #define F_BUFFER 1
 #define F_FREE 2
+#define F_PADDLE 8
 
 #define PIDX()                                   \
   const int i = blockIdx.x * blockDim.x + threadIdx.x; \
@@ -36,6 +37,27 @@ __global__ void k_projection(Dev D) {
 }
 
 // ---------------------------------------------------------------------
+// piston wavemaker (no counterpart in the reference): the paddle's wall
+// particles sit at rest position xs + X and move with velocity U along x.
+// The PPE source term reads a wall neighbour's intermediate velocity (us, vs),
+// which int_vel leaves to walls, so the paddle sets it too.
+// ---------------------------------------------------------------------
+__global__ void k_paddle(Dev D, double X, double U) {
+  PIDX();
+  if (i >= n || D.tid[i] == TID_FLUID || !(D.flags[i] & F_PADDLE)) return;
+  D.x[i] = D.xs[i] + X;
+  D.u[i] = U;
+  D.v[i] = 0.0;
+  D.us[i] = U;
+  D.vs[i] = 0.0;
+}
+__global__ void k_paddle_rest(Dev D, double X) {
+  PIDX();
+  if (i >= n || D.tid[i] == TID_FLUID || !(D.flags[i] & F_PADDLE)) return;
+  D.xs[i] = D.x[i] - X;
+}
+
+// ---------------------------------------------------------------------
 // rebin = cellshift / cellshiftalt (integrator.f90:33-392).
 // A particle moves at most one cell per call, so its destination is fixed by
 // its transfer class: 0 wall, 1 stay, then tn ts tw te tse tne tsw tnw in the
@@ -43,7 +65,8 @@ __global__ void k_projection(Dev D) {
 // the (destination cell, class) bin, stably in the current order, gives the
 // reference's slot order exactly: one counting-sort pass with prefix-summed
 // bin starts (the bins of a cell are consecutive, so its first bin start is
-// the cell start).
+// the cell start).  Wavemaker paddle walls take classes 1-9 by position like
+// fluid, so they follow the fixed walls of their cell.
 // ---------------------------------------------------------------------
 __device__ __forceinline__ int dest_offset_ci(int cls) {
   // row offset of the destination cell relative to the source cell
@@ -75,7 +98,8 @@ __global__ void k_rebin_classify(Dev D, int alt) {
   const DevParams& P = D.P;
   int cls;
   const int key = D.cell[i];
-  if (D.tid[i] != TID_FLUID) {
+  // walls stay in their slots; paddle walls move between cells as fluid does
+  if (D.tid[i] != TID_FLUID && !(D.flags[i] & F_PADDLE)) {
     cls = 0;
   } else {
     int ci, cj;
@@ -96,7 +120,7 @@ __global__ void k_rebin_classify(Dev D, int alt) {
       else if ((y1 >= yb) && (y1 < yt)) cls = 4;
       else if (y1 < yb) cls = 8;
     }
-    if (alt && cls != CLS_DROP) {
+    if (alt && cls != CLS_DROP && D.tid[i] == TID_FLUID) {
       const bool buf = (D.flags[i] & F_BUFFER) != 0;
       if (!((x1 > P.xlcutoff) || buf || ((x1 <= P.xlcutoff) && (y1 <= P.ytcutoff)))) cls = CLS_DROP;
     }

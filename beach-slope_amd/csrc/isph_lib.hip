@@ -48,6 +48,9 @@ struct isph_ctx {
   isph_params P;
   HostZone zones[2];
   bool wcsph = false;      // fused step in the weakly compressible form
+  // piston wavemaker: stroke, period, ramp time and start time of the paddle motion
+  bool wave = false;
+  double wv_stroke = 0.0, wv_period = 1.0, wv_ramp = 0.0, wv_t0 = 0.0;
   bool prof = false;
   KTimer timers[4] = {{"spmv_p"}, {"spmv_s"}, {"spmv_p32"}, {"spmv_s32"}};
   Dev D;
@@ -402,6 +405,10 @@ extern "C" int isph_upload(isph_ctx* c, int64_t n, const double* x, const double
       set_err(c, "particle %lld has tid %d", (long long)i, tid[i]);
       return ISPH_EARG;
     }
+    if (tid[i] == ISPH_TID_FLUID && (flags[i] & ISPH_F_PADDLE)) {
+      set_err(c, "particle %lld is fluid and flagged as a paddle wall", (long long)i);
+      return ISPH_EARG;
+    }
     key[i] = 2 * (int64_t)k + (tid[i] == ISPH_TID_FLUID ? 1 : 0);
   }
   std::vector<int64_t> ord(n);
@@ -429,7 +436,7 @@ extern "C" int isph_upload(isph_ctx* c, int64_t n, const double* x, const double
   std::vector<int32_t> cb(n);
   for (int64_t k = 0; k < n; k++) {
     tb[k] = (uint8_t)tid[ord[k]];
-    fb[k] = flags[ord[k]] & (ISPH_F_BUFFER | ISPH_F_FREE);
+    fb[k] = flags[ord[k]] & (ISPH_F_BUFFER | ISPH_F_FREE | ISPH_F_PADDLE);
     vb[k] = (flags[ord[k]] & ISPH_F_VICINITY) ? 1 : 0;
     cb[k] = (int32_t)(key[ord[k]] / 2);
   }
@@ -1396,6 +1403,75 @@ extern "C" int isph_updateden(isph_ctx* c) {
   return 0;
 }
 
+// ---------------------------------------------------------------------
+// piston wavemaker (no counterpart in the reference)
+// ---------------------------------------------------------------------
+extern "C" int isph_wavemaker_motion(double stroke, double period, double ramp, double t0,
+                                     double t, double* x, double* u) {
+  if (!x || !u || !(period > 0.0) || !(stroke >= 0.0) || !(ramp >= 0.0)) return ISPH_EARG;
+  const double tau = t - t0;
+  if (!(tau > 0.0)) {
+    *x = 0.0;
+    *u = 0.0;
+    return 0;
+  }
+  const double a = 0.5 * stroke;
+  const double w = (2.0 * M_PI) / period;
+  double r = 1.0, rp = 0.0;
+  if (ramp > 0.0 && tau < ramp) {
+    r = tau / ramp;
+    rp = 1.0 / ramp;
+  }
+  const double sn = std::sin(w * tau), cs = std::cos(w * tau);
+  *x = (a * r) * sn;
+  *u = a * (((r * w) * cs) + (rp * sn));
+  return 0;
+}
+
+// the paddle at the motion's value for time t
+static int paddle_to(isph_ctx* c, double t) {
+  double X, U;
+  if (isph_wavemaker_motion(c->wv_stroke, c->wv_period, c->wv_ramp, c->wv_t0, t, &X, &U))
+    return ISPH_EARG;
+  LAUNCH(k_paddle, c->n_hi, c->D, X, U);
+  return 0;
+}
+
+extern "C" int isph_set_wavemaker(isph_ctx* c, double stroke, double period, double ramp,
+                                  double t0) {
+  if (!c) return ISPH_EARG;
+  double X, U;
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  if (isph_wavemaker_motion(stroke, period, ramp, t0, s.t, &X, &U)) {
+    set_err(c, "wavemaker needs stroke >= 0, period > 0 and ramp >= 0 (got %g, %g, %g)",
+            stroke, period, ramp);
+    return ISPH_EARG;
+  }
+  set_nhi(c, s.n);
+  LAUNCH(k_paddle_rest, c->n_hi, c->D, X);
+  c->wv_stroke = stroke;
+  c->wv_period = period;
+  c->wv_ramp = ramp;
+  c->wv_t0 = t0;
+  c->wave = true;
+  return 0;
+}
+
+extern "C" int isph_wavemaker(isph_ctx* c) {
+  if (!c) return ISPH_EARG;
+  if (!c->wave) {
+    set_err(c, "wavemaker is not set");
+    return ISPH_ESTATE;
+  }
+  DevScalars s;
+  int rc = read_scalars(c, &s);
+  if (rc) return rc;
+  set_nhi(c, s.n);
+  return paddle_to(c, s.t);
+}
+
 // the insertion clock of main.f90:217-235 (told = t - dt)
 static int buffer_clock(isph_ctx* c, double told, double t) {
   for (int z = 0; z < 2; z++) {
@@ -1430,6 +1506,10 @@ static int step_pre(isph_ctx* c) {
   rc = buffer_clock(c, s.t, s.t + s.dt);  // the device's t + dt, the same double sum
   if (rc) return rc;
   LAUNCH(k_projection, c->n_hi, c->D);
+  if (c->wave) {
+    rc = paddle_to(c, s.t + s.dt);  // the device's t + dt, the same double sum
+    if (rc) return rc;
+  }
   rc = rebin(c, 0);
   if (rc) return rc;
   rc = neighbour_nocheck(c);

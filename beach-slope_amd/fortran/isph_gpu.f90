@@ -57,6 +57,7 @@ module isph_gpu
   public :: isph_init_state, isph_step, isph_footprint_bytes
   public :: isph_set_buffer, isph_buffer_update
   public :: isph_set_wcsph, isph_presstoden, isph_dentopress, isph_calmasscons, isph_updateden
+  public :: isph_wavemaker_motion, isph_set_wavemaker, isph_wavemaker
   public :: isph_error_message
 
   interface
@@ -233,6 +234,25 @@ module isph_gpu
       type(c_ptr), value :: ctx
     end function
     integer(c_int) function isph_updateden(ctx) bind(C, name="isph_updateden")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    ! piston wavemaker (no counterpart in the reference): the ISPH_F_PADDLE
+    ! (flag bit 8) walls move by X(t); isph_step moves them after projection
+    integer(c_int) function isph_wavemaker_motion(stroke, period, ramp, t0, t, x, u) &
+        bind(C, name="isph_wavemaker_motion")
+      import :: c_int, c_double
+      real(c_double), value :: stroke, period, ramp, t0, t
+      real(c_double), intent(out) :: x, u
+    end function
+    integer(c_int) function isph_set_wavemaker(ctx, stroke, period, ramp, t0) &
+        bind(C, name="isph_set_wavemaker")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: stroke, period, ramp, t0
+    end function
+    integer(c_int) function isph_wavemaker(ctx) bind(C, name="isph_wavemaker")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function

@@ -1,6 +1,7 @@
 ! isph_main.f90 -- Fortran driver of the device ISPH pipeline.
 !
 !   isph_driver <case.bin> <state_out.bin> [fused] [snapshots=DIR]
+!               [wave=STROKE,PERIOD,RAMP,T0]
 !
 ! Reads a case (run parameters, particles, porous solids, end time and
 ! display interval) from a binary stream file, runs the production loop of
@@ -8,6 +9,11 @@
 ! subroutine in the reference's order, prints one line per display interval
 ! and writes the final particle state to <state_out.bin>.  With the argument
 ! "fused" each iteration is one isph_step call instead.
+!
+! With wave=STROKE,PERIOD,RAMP,T0 the walls flagged as paddle (flag bit 8)
+! are a piston wavemaker (isph_set_wavemaker, after the upload); the
+! per-subroutine loop moves the paddle after projection (isph_wavemaker), as
+! isph_step does.  The reference has no wavemaker.
 !
 ! With snapshots=DIR the driver writes the reference's output files under DIR
 ! (whose subdirectories fluid/, buffer/, boundary/ and ghost/ must exist, as
@@ -51,11 +57,13 @@ program isph_main
   logical :: fused
   integer(c_int64_t) :: nsteps
   character(len=4096) :: snapdir, arg
-  logical :: snaps
+  logical :: snaps, wave
+  real(c_double) :: wv(4)
   integer :: ia
 
   if (command_argument_count() < 2) then
-    write(*, '(a)') 'usage: isph_driver <case.bin> <state_out.bin> [fused] [snapshots=DIR]'
+    write(*, '(a)') 'usage: isph_driver <case.bin> <state_out.bin> [fused] [snapshots=DIR] ' // &
+                    '[wave=STROKE,PERIOD,RAMP,T0]'
     stop 2
   end if
   call get_command_argument(1, inpath)
@@ -63,6 +71,7 @@ program isph_main
   fused = .false.
   snaps = .false.
   snapdir = ''
+  wave = .false.
   do ia = 3, command_argument_count()
     call get_command_argument(ia, arg)
     if (trim(arg) == 'fused') then
@@ -70,6 +79,13 @@ program isph_main
     else if (arg(1:10) == 'snapshots=') then
       snaps = .true.
       snapdir = arg(11:)
+    else if (arg(1:5) == 'wave=') then
+      read(arg(6:), *, iostat=ios) wv
+      if (ios /= 0) then
+        write(*, '(2a)') 'cannot read four numbers from ', trim(arg)
+        stop 2
+      end if
+      wave = .true.
     else
       write(*, '(2a)') 'unknown argument ', trim(arg)
       stop 2
@@ -107,6 +123,7 @@ program isph_main
   call check(isph_upload(ctx, n, x, y, vx, vy, p, m, rho, rho0, vol0, con, tid, flags), &
              'isph_upload')
   if (ns > 0) call check(isph_upload_solids(ctx, ns, sx, sy, sm, srho), 'isph_upload_solids')
+  if (wave) call check(isph_set_wavemaker(ctx, wv(1), wv(2), wv(3), wv(4)), 'isph_set_wavemaker')
   call check(isph_init_state(ctx), 'isph_init_state')
 
   if (snaps) then
@@ -131,6 +148,7 @@ program isph_main
         call check(isph_scalarupdate(ctx, sc%dtsol), 'scalarupdate')
       end do
       call check(isph_projection(ctx), 'projection')
+      if (wave) call check(isph_wavemaker(ctx), 'wavemaker')
       call check(isph_cellshift(ctx), 'cellshift')
       call check(isph_resetid(ctx), 'resetid')
       call check(isph_neighbour(ctx), 'neighbour')

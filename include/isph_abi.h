@@ -46,6 +46,9 @@ enum { ISPH_TID_FIXED = 1, ISPH_TID_GHOST = 2, ISPH_TID_FLUID = 3 };
 
 /* Flag bits carried per particle (the logicals of particle.f90:42-43). */
 enum { ISPH_F_BUFFER = 1, ISPH_F_FREE = 2, ISPH_F_VICINITY = 4 };
+/* A wall particle (tid 1 or 2) of the wavemaker paddle; it has no
+ * counterpart in the reference, whose walls never move. */
+enum { ISPH_F_PADDLE = 8 };
 
 /* Maximum neighbours per particle (the reference's per-slot capacity
  * `fplist`, memory.f90:25).  Exceeding it is ISPH_ECAP. */
@@ -213,6 +216,30 @@ int isph_presstoden(isph_ctx *ctx);
 int isph_dentopress(isph_ctx *ctx);
 int isph_calmasscons(isph_ctx *ctx);
 int isph_updateden(isph_ctx *ctx);
+
+/* ---- piston wavemaker (no counterpart in the reference) ----
+ * The paddle is the wall particles flagged ISPH_F_PADDLE.  It moves along x
+ * by X(t) from its rest position, with velocity U(t):
+ *   tau = t - t0;  a = stroke / 2;  w = 2 pi / period
+ *   r   = ramp > 0 and tau < ramp ? tau / ramp : 1   (0 for tau <= 0)
+ *   r'  = ramp > 0 and 0 < tau < ramp ? 1 / ramp : 0
+ *   X   = (a r) sin(w tau)
+ *   U   = a ((r w) cos(w tau) + r' sin(w tau))
+ * isph_wavemaker_motion evaluates X and U at time t (host arithmetic, in
+ * this order of operations).
+ * isph_set_wavemaker records each paddle particle's rest position as its
+ * current x minus X at the context's current time, and turns the motion on
+ * (stroke 0 holds the paddle at rest).  isph_step then runs isph_wavemaker
+ * after projection in every step.
+ * isph_wavemaker moves the paddle to X at the context's current time:
+ * x = rest + X, u = U, v = 0, and the intermediate velocity (which the PPE
+ * source term reads for wall neighbours) us = U, vs = 0.  The rebinning classifies paddle particles by
+ * position as it does fluid (the cellshiftalt cut-off aside), so they move
+ * between cells; everything else treats them as the walls they are. */
+int isph_wavemaker_motion(double stroke, double period, double ramp, double t0, double t,
+                          double *x, double *u);
+int isph_set_wavemaker(isph_ctx *ctx, double stroke, double period, double ramp, double t0);
+int isph_wavemaker(isph_ctx *ctx);
 
 /* The two halves of ppesolve, for phase-by-phase parity:
  * assembly (isph.f90:123-456) and solve + pressure scatter (isph.f90:462-486). */

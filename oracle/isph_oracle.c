@@ -48,6 +48,7 @@
 #define KCAP ISPH_NBR_CAP
 #define RCAP (ISPH_NBR_CAP + 2)
 #define RED_BLOCK 256
+#define OR_PI 3.14159265358979323846 /* M_PI of <math.h> */
 
 /* an inflow/outflow buffer zone (internal.f90:447-586): the template
  * particles buffer1/buffer2, their cells (the entry cells), the values an
@@ -70,6 +71,10 @@ typedef struct ostate {
    * the fused step to the weakly compressible form */
   double c0, gam, delt, swl;
   int wcsph;
+  /* piston wavemaker: stroke, period, ramp time, start time; the paddle's
+   * rest positions live in xs */
+  double wv_stroke, wv_period, wv_ramp, wv_t0;
+  int wave;
   double *tden;
   int64_t n, cap, ns, scap;
   int32_t ncell;
@@ -308,7 +313,7 @@ int or_upload(ostate *s, int64_t n, const double *x, const double *y, const doub
   for (int64_t i = 0; i < n; i++) {
     int ci, cj;
     int k = locate(s, x[i], y[i], &ci, &cj);
-    if (k < 0) {
+    if (k < 0 || (tid[i] == ISPH_TID_FLUID && (flags[i] & ISPH_F_PADDLE))) {
       free(key);
       free(cnt);
       return ISPH_EARG;
@@ -408,7 +413,8 @@ static void or_rebin(ostate *s, int alt) {
   const int celly = s->P.celly, cellx = s->P.cellx;
   /* classification pass: one transfer class per fluid particle */
   for (int64_t i = 0; i < s->n; i++) {
-    if (s->tid[i] != ISPH_TID_FLUID) {
+    /* walls keep their slots; wavemaker paddle walls move as fluid does */
+    if (s->tid[i] != ISPH_TID_FLUID && !(s->flags[i] & ISPH_F_PADDLE)) {
       s->cls[i] = C_WALL;
       continue;
     }
@@ -429,7 +435,7 @@ static void or_rebin(ostate *s, int alt) {
       else if ((y1 >= yb) && (y1 < yt)) c = C_TW;
       else if (y1 < yb) c = C_TSW;
     }
-    if (alt && c != C_DROP) {
+    if (alt && c != C_DROP && s->tid[i] == ISPH_TID_FLUID) {
       int buf = (s->flags[i] & ISPH_F_BUFFER) != 0;
       if (!((x1 > s->P.xlcutoff) || buf || ((x1 <= s->P.xlcutoff) && (y1 <= s->P.ytcutoff))))
         c = C_DROP;
@@ -1412,6 +1418,67 @@ void or_updateden(ostate *s) {
     if (s->tid[i] > 2) s->rho[i] = s->tden[i];
 }
 
+/* ------------------------------------------------------------------ */
+/* piston wavemaker (no counterpart in the reference)                   */
+/* ------------------------------------------------------------------ */
+
+/* X(t) = (a r) sin(w tau), U = dX/dt with the linear start-up ramp r(tau):
+ * the paddle displacement and velocity of a piston wavemaker of stroke 2a
+ * (Dean & Dalrymple, Water Wave Mechanics, sec. 6.2). */
+int or_wavemaker_motion(double stroke, double period, double ramp, double t0, double t,
+                        double *x, double *u) {
+  if (!x || !u || !(period > 0.0) || !(stroke >= 0.0) || !(ramp >= 0.0)) return ISPH_EARG;
+  double tau = t - t0;
+  if (!(tau > 0.0)) {
+    *x = 0.0;
+    *u = 0.0;
+    return 0;
+  }
+  double a = 0.5 * stroke, w = (2.0 * OR_PI) / period, r = 1.0, rp = 0.0;
+  if (ramp > 0.0 && tau < ramp) {
+    r = tau / ramp;
+    rp = 1.0 / ramp;
+  }
+  double sn = sin(w * tau), cs = cos(w * tau);
+  *x = (a * r) * sn;
+  *u = a * (((r * w) * cs) + (rp * sn));
+  return 0;
+}
+
+static int paddle_to(ostate *s, double t) {
+  double X, U;
+  if (or_wavemaker_motion(s->wv_stroke, s->wv_period, s->wv_ramp, s->wv_t0, t, &X, &U))
+    return ISPH_EARG;
+  for (int64_t i = 0; i < s->n; i++) {
+    if (s->tid[i] == ISPH_TID_FLUID || !(s->flags[i] & ISPH_F_PADDLE)) continue;
+    s->x[i] = s->xs[i] + X;
+    s->u[i] = U;
+    s->v[i] = 0.0;
+    /* the PPE source term reads a wall neighbour's intermediate velocity */
+    s->us[i] = U;
+    s->vs[i] = 0.0;
+  }
+  return 0;
+}
+
+int or_set_wavemaker(ostate *s, double stroke, double period, double ramp, double t0) {
+  double X, U;
+  if (or_wavemaker_motion(stroke, period, ramp, t0, s->t, &X, &U)) return ISPH_EARG;
+  for (int64_t i = 0; i < s->n; i++)
+    if (s->tid[i] != ISPH_TID_FLUID && (s->flags[i] & ISPH_F_PADDLE)) s->xs[i] = s->x[i] - X;
+  s->wv_stroke = stroke;
+  s->wv_period = period;
+  s->wv_ramp = ramp;
+  s->wv_t0 = t0;
+  s->wave = 1;
+  return 0;
+}
+
+int or_wavemaker(ostate *s) {
+  if (!s->wave) return ISPH_ESTATE;
+  return paddle_to(s, s->t);
+}
+
 /* the insertion clock of main.f90:217-235, told = t - dt */
 static int buffer_clock(ostate *s, double told) {
   for (int z = 0; z < 2; z++) {
@@ -1440,6 +1507,7 @@ int or_step(ostate *s, int nsteps, isph_solve_stats *st) {
     int brc = buffer_clock(s, told);
     if (brc) return brc;
     or_projection(s);
+    if (s->wave) paddle_to(s, s->t);
     or_cellshift(s);
     int rc = or_neighbour(s);
     if (rc) return rc;

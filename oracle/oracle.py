@@ -107,6 +107,9 @@ def lib():
         L.or_set_buffer.argtypes = [vp, C.c_int, i64] + [vp] * 5 + [d] * 7
         L.or_buffer_update.argtypes = [vp, C.c_int]
         L.or_set_wcsph.argtypes = [vp, d, d, d, d, C.c_int]
+        L.or_wavemaker_motion.argtypes = [d, d, d, d, d, C.POINTER(d), C.POINTER(d)]
+        L.or_set_wavemaker.argtypes = [vp, d, d, d, d]
+        L.or_wavemaker.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -150,6 +153,12 @@ class Oracle:
                                     _ptr(sol["m"]), _ptr(sol["rho"]))
             if rc != 0:
                 raise RuntimeError(f"or_upload_solids failed: {rc}")
+        wave = case.meta.get("wavemaker")
+        if wave:
+            rc = L.or_set_wavemaker(self.s, wave["stroke"], wave["period"], wave["ramp"],
+                                    wave["t0"])
+            if rc != 0:
+                raise RuntimeError(f"or_set_wavemaker failed: {rc}")
 
     def __del__(self):
         try:
@@ -220,6 +229,10 @@ class Oracle:
     def buffer_update(self, zone: int) -> int:
         return lib().or_buffer_update(self.s, zone)
 
+    def set_wavemaker(self, stroke: float, period: float, ramp: float = 0.0,
+                      t0: float = 0.0) -> int:
+        return lib().or_set_wavemaker(self.s, stroke, period, ramp, t0)
+
     def step(self, nsteps: int = 1):
         st = SolveStats()
         rc = lib().or_step(self.s, nsteps, C.byref(st))
@@ -232,3 +245,12 @@ class Oracle:
 
 def threads() -> int:
     return int(lib().or_threads())
+
+
+def wavemaker_motion(stroke: float, period: float, ramp: float, t0: float, t: float):
+    """(X, U) of the oracle's paddle motion at time t."""
+    x, u = C.c_double(), C.c_double()
+    rc = lib().or_wavemaker_motion(stroke, period, ramp, t0, t, C.byref(x), C.byref(u))
+    if rc != 0:
+        raise ValueError(f"or_wavemaker_motion: {rc}")
+    return x.value, u.value
