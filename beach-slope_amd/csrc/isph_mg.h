@@ -102,6 +102,32 @@ __device__ __forceinline__ int mg_nbr_at(const MgLevel& L, int ci, int cj, int k
   return b * L.ny + a;
 }
 
+// the nine neighbours of cell I = (ci, cj) in k order (row offset k % 3 - 1,
+// column offset k / 3 - 1), -1 outside the grid: mg_nbr_at for every k with
+// the four edge tests shared
+__device__ __forceinline__ void mg_nbr9(const MgLevel& L, int ci, int cj, int I, int* N) {
+  const bool u = ci > 0, d = ci + 1 < L.ny, l = cj > 0, r = cj + 1 < L.nx;
+  const int c0 = I - L.ny, c2 = I + L.ny;
+  N[0] = (l && u) ? c0 - 1 : -1;
+  N[1] = l ? c0 : -1;
+  N[2] = (l && d) ? c0 + 1 : -1;
+  N[3] = u ? I - 1 : -1;
+  N[4] = I;
+  N[5] = d ? I + 1 : -1;
+  N[6] = (r && u) ? c2 - 1 : -1;
+  N[7] = r ? c2 : -1;
+  N[8] = (r && d) ? c2 + 1 : -1;
+}
+
+// mg_parent_at for every k: the level-C cell holding neighbour k of (ci, cj)
+__device__ __forceinline__ void mg_par9(const MgLevel& C, int ci, int cj, int* P) {
+  const int r0 = (ci - 1) >> 1, r1 = ci >> 1, r2 = (ci + 1) >> 1;
+  const int c0 = ((cj - 1) >> 1) * C.ny, c1 = (cj >> 1) * C.ny, c2 = ((cj + 1) >> 1) * C.ny;
+  P[0] = c0 + r0; P[1] = c0 + r1; P[2] = c0 + r2;
+  P[3] = c1 + r0; P[4] = c1 + r1; P[5] = c1 + r2;
+  P[6] = c2 + r0; P[7] = c2 + r1; P[8] = c2 + r2;
+}
+
 // the cell of level C = l+1 holding neighbour k of cell (ci, cj) of level l
 __device__ __forceinline__ int mg_parent_at(const MgLevel& C, int ci, int cj, int k) {
   return ((cj + (k / 3) - 1) >> 1) * C.ny + ((ci + (k % 3) - 1) >> 1);
@@ -369,11 +395,11 @@ __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
   if (in) {
     const int I = cj * L.ny + ci;
     float acc = 0.0f;
+    int Nb[9];
+    mg_nbr9(L, ci, cj, I, Nb);
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const int N = mg_nbr_at(L, ci, cj, k);
-      if (N >= 0) acc = acc + st[k * L.n + I] * (w * b[N] / st[4 * L.n + N]);
-    }
+    for (int k = 0; k < 9; k++)
+      if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (w * b[Nb[k]] / st[4 * L.n + Nb[k]]);
     x[I] = w * b[I] / st[4 * L.n + I];
     res = b[I] - acc;
   }
@@ -408,11 +434,12 @@ __device__ __forceinline__ void mg_lvl_up(const Dev& D, const MgDev& G, int l, i
   const float* __restrict__ xc = (l == G.nmb ? G.xb : G.rb) + C.off;
   const int cj = I / L.ny, ci = I - cj * L.ny;
   float acc = 0.0f;
+  int Nb[9], Pa[9];
+  mg_nbr9(L, ci, cj, I, Nb);
+  mg_par9(C, ci, cj, Pa);
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int N = mg_nbr_at(L, ci, cj, k);
-    if (N >= 0) acc = acc + st[k * L.n + I] * (x[N] + c * xc[mg_parent_at(C, ci, cj, k)]);
-  }
+  for (int k = 0; k < 9; k++)
+    if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (x[Nb[k]] + c * xc[Pa[k]]);
   const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
   const float out = x1 + w * (b[I] - acc) / st[4 * L.n + I];
   G.rb[L.off + I] = out;
@@ -519,11 +546,11 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
         const int I = t + q * T;
         if (I < L.n) {
           float acc = 0.0f;
+          int Nb[9];
+          mg_nbr9(L, ci0[q], cj0[q], I, Nb);
 #pragma unroll
-          for (int k = 0; k < 9; k++) {
-            const int N = mg_nbr_at(L, ci0[q], cj0[q], k);
-            if (N >= 0) acc = acc + st0[q][k] * x[N];
-          }
+          for (int k = 0; k < 9; k++)
+            if (Nb[k] >= 0) acc = acc + st0[q][k] * x[Nb[k]];
           r[I] = b[I] - acc;
         }
       }
@@ -533,11 +560,11 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
       for (int I = t; I < L.n; I += T) {
         const int cj = I / L.ny, ci = I - cj * L.ny;
         float acc = 0.0f;
+        int Nb[9];
+        mg_nbr9(L, ci, cj, I, Nb);
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-          const int N = mg_nbr_at(L, ci, cj, k);
-          if (N >= 0) acc = acc + st[k * L.n + I] * x[N];
-        }
+        for (int k = 0; k < 9; k++)
+          if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * x[Nb[k]];
         r[I] = b[I] - acc;
       }
     }
@@ -584,11 +611,12 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
         if (I < L.n) {
           const int ci = ci0[q], cj = cj0[q];
           float acc = 0.0f;
+          int Nb[9], Pa[9];
+          mg_nbr9(L, ci, cj, I, Nb);
+          mg_par9(C, ci, cj, Pa);
 #pragma unroll
-          for (int k = 0; k < 9; k++) {
-            const int N = mg_nbr_at(L, ci, cj, k);
-            if (N >= 0) acc = acc + st0[q][k] * (x[N] + c * xc[mg_parent_at(C, ci, cj, k)]);
-          }
+          for (int k = 0; k < 9; k++)
+            if (Nb[k] >= 0) acc = acc + st0[q][k] * (x[Nb[k]] + c * xc[Pa[k]]);
           const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
           out[I] = x1 + w * (b[I] - acc) / st0[q][4];
         }
@@ -597,11 +625,12 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
       for (int I = t; I < L.n; I += T) {
         const int cj = I / L.ny, ci = I - cj * L.ny;
         float acc = 0.0f;
+        int Nb[9], Pa[9];
+        mg_nbr9(L, ci, cj, I, Nb);
+        mg_par9(C, ci, cj, Pa);
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-          const int N = mg_nbr_at(L, ci, cj, k);
-          if (N >= 0) acc = acc + st[k * L.n + I] * (x[N] + c * xc[mg_parent_at(C, ci, cj, k)]);
-        }
+        for (int k = 0; k < 9; k++)
+          if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (x[Nb[k]] + c * xc[Pa[k]]);
         const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
         out[I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
       }
