@@ -20,7 +20,7 @@ enum {
   BK_ROWS = 0,                           // particle rows
   BK_CELL1 = 1,                          // level-1 cells
   BK_DOWN = 2,                           // + l-1: 4 x level-(l+1) aggregates, l <= nmb
-  BK_UP = 2 + MG_MAX_LEVELS,             // + l-1: level-l cells, l <= nmb
+  BK_UP = 2 + MG_MAX_LEVELS,             // + l-1: level-l cells (x 4 on level 1), l <= nmb
   BK_COARSEN = 2 + 2 * MG_MAX_LEVELS,    // + l-1: level-(l+1) cells, l < nlev
   BK_NKIND = 2 + 3 * MG_MAX_LEVELS
 };

@@ -847,7 +847,7 @@ static int bh_make(isph_ctx* const* ctxs, int S, BatchHost** out) {
     put(BK_CELL1, grid_of(G.lev[1].n));
     for (int l = 1; l < MG_MAX_LEVELS; l++) {
       put(BK_DOWN + l - 1, (l <= G.nmb) ? grid_of(4 * G.lev[l + 1].n) : 0);
-      put(BK_UP + l - 1, (l <= G.nmb) ? grid_of(G.lev[l].n) : 0);
+      put(BK_UP + l - 1, (l <= G.nmb) ? grid_of((l == 1 ? 4 : 1) * G.lev[l].n) : 0);
       put(BK_COARSEN + l - 1, (l < G.nlev) ? grid_of(G.lev[l + 1].n) : 0);
     }
   }

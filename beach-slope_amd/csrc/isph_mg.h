@@ -350,24 +350,33 @@ __device__ __forceinline__ void mg_restrict0_cell(const Dev& D, const MgDev& G,
   const MgLevel L = G.lev[1];
   if (*G.done || I >= L.n) return;
   float s = 0.0f;
-  const int i1 = D.cstart[I + 1];
-  for (int i = D.cstart[I]; i < i1; i++) s = s + r0[i];
+  const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
+  for (int b0 = i0; b0 < i1; b0 += 8) {  // eight loads in flight, then the sums in row order
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = (b0 + u < i1) ? r0[b0 + u] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (b0 + u < i1) s = s + v[u];
+  }
   G.bb[L.off + I] = s;
 }
 
 // level 0 rows of level-1 cell I: x1 = x0 + c e1, in place of x0 (e1 is the
-// cell's final level-1 correction); run by the pass that forms e1
-__device__ __forceinline__ void mg_prolong_cell(const Dev& D, const MgDev& G, int I, float e1) {
+// cell's final level-1 correction); run by the pass that forms e1.  Lane q
+// of `lanes` lanes sharing the cell takes rows i0 + q, i0 + q + lanes, ...
+__device__ __forceinline__ void mg_prolong_cell(const Dev& D, const MgDev& G, int I, float e1,
+                                                int q = 0, int lanes = 1) {
   const float ce = G.cscale * e1;
   float* __restrict__ x0 = G.x0;
-  const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
-  for (int b0 = i0; b0 < i1; b0 += 8) {
+  const int i0 = D.cstart[I] + q, i1 = D.cstart[I + 1];
+  for (int b0 = i0; b0 < i1; b0 += 8 * lanes) {
     float v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = (b0 + u < i1) ? x0[b0 + u] : 0.0f;
+    for (int u = 0; u < 8; u++) v[u] = (b0 + u * lanes < i1) ? x0[b0 + u * lanes] : 0.0f;
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (b0 + u < i1) x0[b0 + u] = v[u] + ce;
+      if (b0 + u * lanes < i1) x0[b0 + u * lanes] = v[u] + ce;
   }
 }
 
@@ -422,10 +431,12 @@ __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
 // values formed on the fly.  The level-(l+1) correction is in xb when that
 // level ran in the single workgroup (l == nmb), else in rb.  The result goes
 // to the level's residual slot (free by then), where the next finer level
-// reads it; on level 1 the lane also adds it to the x0 of its cell's rows.
-__device__ __forceinline__ void mg_lvl_up(const Dev& D, const MgDev& G, int l, int I) {
+// reads it.  On level 1 four lanes share a cell (g = 4 I + q): each forms
+// the same result and adds it to a quarter of the x0 of the cell's rows.
+__device__ __forceinline__ void mg_lvl_up(const Dev& D, const MgDev& G, int l, int g) {
   if (*G.done || l > G.nmb) return;
   const MgLevel L = G.lev[l], C = G.lev[l + 1];
+  const int I = (l == 1) ? (g >> 2) : g;
   if (I >= L.n) return;
   const float w = G.w, c = G.cscale;
   const float* st = G.sten + 9 * L.off;
@@ -442,8 +453,13 @@ __device__ __forceinline__ void mg_lvl_up(const Dev& D, const MgDev& G, int l, i
     if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (x[Nb[k]] + c * xc[Pa[k]]);
   const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
   const float out = x1 + w * (b[I] - acc) / st[4 * L.n + I];
-  G.rb[L.off + I] = out;
-  if (l == 1) mg_prolong_cell(D, G, I, out);
+  if (l != 1) {
+    G.rb[L.off + I] = out;
+    return;
+  }
+  const int q = g & 3;
+  if (q == 0) G.rb[L.off + I] = out;
+  mg_prolong_cell(D, G, I, out, q, 4);
 }
 
 // levels l0 = nmb+1..nlev in one workgroup: down sweeps, dense solve, up
