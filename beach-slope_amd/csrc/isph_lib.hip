@@ -169,7 +169,9 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   if (precision) per += 2 * 4 * ((ROW_CAP + 1) / 2) + 5 * 4 + 2 * 4 + 9 * 8;
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
   b += 8 * 2 * (n_cap / 256 + 2);               // reduction partials
-  if (precision) b += 4 * 12 * ncell + 8 * MG_DENSE_MAX * MG_DENSE_MAX;  // cell levels
+  // cell levels: stencil, x, b, r and 1/diagonal per unknown, the levels
+  // holding 4/3 of the cell count
+  if (precision) b += 4 * 13 * ((4 * ncell) / 3 + MG_MAX_LEVELS) + 8 * MG_DENSE_MAX * MG_DENSE_MAX;
   return b + (1 << 20);
 }
 
@@ -255,6 +257,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
     rc |= dalloc(c, &G.xb, off);
     rc |= dalloc(c, &G.bb, off);
     rc |= dalloc(c, &G.rb, off);
+    rc |= dalloc(c, &G.rdg, off);
     rc |= dalloc(c, &G.dense_inv, MG_DENSE_MAX * MG_DENSE_MAX);
     rc |= dalloc(c, &G.rowst, 9 * N);
     rc |= dalloc(c, &G.d0, N);

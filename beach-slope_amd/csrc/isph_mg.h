@@ -71,6 +71,7 @@ struct MgDev {
   float* xb;                   // per-level solution pool
   float* bb;                   // per-level right-hand side pool
   float* rb;                   // per-level residual pool
+  float* rdg;                  // per-level 1 / stencil diagonal (smoother weights)
   double* dense_inv;           // MG_DENSE_MAX^2 explicit inverse of the coarsest level
   double* rowst;               // 9 * cap per-row partial level-1 stencils (setup scratch)
   float* d0;                   // level 0: 1 / (sum of the row's entries in column i)
@@ -80,17 +81,17 @@ struct MgDev {
 };
 
 // LDS bytes the coarse workgroup needs when it runs levels l0..nlev: the
-// dense level's inverse (FP64), x, b and r of every level, the stencils of
-// levels l0+1..nlev-1 and the diagonal of level l0 (whose stencil rows are
-// held in registers)
+// dense level's inverse (FP64), x, b and r of every level, and the stencils
+// and diagonal reciprocals of levels l0+1..nlev-1 (level l0's are held in
+// registers)
 __host__ __device__ inline int64_t mg_coarse_lds_bytes(const MgDev& G, int l0) {
   int64_t v = 0, st = 0;
   for (int l = l0; l <= G.nlev; l++) {
     v += G.lev[l].n;
-    if (l > l0 && l < G.nlev) st += 9 * (int64_t)G.lev[l].n;
+    if (l > l0 && l < G.nlev) st += 10 * (int64_t)G.lev[l].n;
   }
   const int64_t nd = G.lev[G.nlev].n;
-  return 8 * nd * nd + 4 * (3 * v + st + G.lev[l0].n);
+  return 8 * nd * nd + 4 * (3 * v + st);
 }
 
 // neighbour k of cell (ci, cj) of level L (row offset k % 3 - 1, column
@@ -219,6 +220,7 @@ __device__ __forceinline__ void mg_stencil1_cell(const Dev& D, const MgDev& G, i
     for (int i = i0; i < i1; i++) s = s + G.rowst[q * cap + i];
     if (q == 4 && i1 == i0) s = 1.0;
     st[q * L.n + I] = (float)s;
+    if (q == 4) G.rdg[L.off + I] = 1.0f / (float)s;
   }
 }
 
@@ -254,6 +256,7 @@ __device__ __forceinline__ void mg_coarsen_cell(const MgDev& G, int l, int I) {
   float* sc = G.sten + 9 * C.off;
 #pragma unroll
   for (int k = 0; k < 9; k++) sc[k * C.n + I] = s[k];
+  G.rdg[C.off + I] = 1.0f / s[4];
 }
 
 // Coarsest level (one workgroup): dense matrix and its explicit inverse by
@@ -395,6 +398,7 @@ __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
   if (J >= C.n) return;
   const float w = G.w;
   const float* st = G.sten + 9 * L.off;
+  const float* rd = G.rdg + L.off;
   const float* b = G.bb + L.off;
   float* x = G.xb + L.off;
   const int CJ = J / C.ny, CI = J - CJ * C.ny;
@@ -408,8 +412,8 @@ __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
     mg_nbr9(L, ci, cj, I, Nb);
 #pragma unroll
     for (int k = 0; k < 9; k++)
-      if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (w * b[Nb[k]] / st[4 * L.n + Nb[k]]);
-    x[I] = w * b[I] / st[4 * L.n + I];
+      if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * ((w * b[Nb[k]]) * rd[Nb[k]]);
+    x[I] = (w * b[I]) * rd[I];
     res = b[I] - acc;
   }
   const int lane = threadIdx.x & 63, base = lane & ~3;
@@ -452,7 +456,7 @@ __device__ __forceinline__ void mg_lvl_up(const Dev& D, const MgDev& G, int l, i
   for (int k = 0; k < 9; k++)
     if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (x[Nb[k]] + c * xc[Pa[k]]);
   const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
-  const float out = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+  const float out = x1 + (w * (b[I] - acc)) * G.rdg[L.off + I];
   if (l != 1) {
     G.rb[L.off + I] = out;
     return;
@@ -493,13 +497,14 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
   float* Bv = X + S;
   float* R = X + 2 * S;
   float* ST = X + 3 * S;                          // level l's stencil at 9 (off - stb)
-  float* DG0 = ST + nst;                          // level l0's diagonal
+  float* RD = ST + nst;                           // level l's 1 / diagonal at off - stb
+  const int nrd = nst / 9;
   // every load of the set-up is issued before the first store: one memory
   // latency rather than one per array
   const int dn = *G.done;
   const float* sg0 = G.sten + 9 * L0.off;
   const float* sgs = G.sten + 9 * stb;
-  float st0[MG_CO_PER][9], bq[MG_CO_PER];
+  float st0[MG_CO_PER][9], bq[MG_CO_PER], rd0[MG_CO_PER];
   double dv[(MG_DENSE_MAX * MG_DENSE_MAX + MG_CO_THREADS - 1) / MG_CO_THREADS];
   float sv[MG_CO_STV];
   int ci0[MG_CO_PER], cj0[MG_CO_PER];
@@ -509,7 +514,9 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
 #pragma unroll
     for (int k = 0; k < 9; k++) st0[q][k] = (I < L0.n) ? sg0[k * L0.n + I] : 0.0f;
     bq[q] = (I < L0.n) ? G.bb[base + I] : 0.0f;
+    rd0[q] = (I < L0.n) ? G.rdg[base + I] : 0.0f;
   }
+  const float rv = (t < nrd) ? G.rdg[stb + t] : 0.0f;
 #pragma unroll
   for (int u = 0; u < (int)(sizeof dv / sizeof dv[0]); u++) {
     const int e = t + u * T;
@@ -525,7 +532,6 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
     const int I = t + q * T;
     if (I < L0.n) {
       Bv[I] = bq[q];
-      DG0[I] = st0[q][4];
     }
     cj0[q] = I / L0.ny;
     ci0[q] = I - cj0[q] * L0.ny;
@@ -541,6 +547,8 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
     if (e < nst) ST[e] = sv[u];
   }
   for (int e = t + MG_CO_STV * T; e < nst; e += T) ST[e] = sgs[e];
+  if (t < nrd) RD[t] = rv;
+  for (int e = t + T; e < nrd; e += T) RD[e] = G.rdg[stb + e];
   if (dn) return;  // the whole workgroup: the flag is per scenario
   __syncthreads();
   // down: x = w b / diag; r = b - A x; r restricted to the next level's b
@@ -554,7 +562,7 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
 #pragma unroll
       for (int q = 0; q < MG_CO_PER; q++) {
         const int I = t + q * T;
-        if (I < L.n) x[I] = w * b[I] / st0[q][4];
+        if (I < L.n) x[I] = (w * b[I]) * rd0[q];
       }
       __syncthreads();
 #pragma unroll
@@ -571,7 +579,8 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
         }
       }
     } else {
-      for (int I = t; I < L.n; I += T) x[I] = w * b[I] / st[4 * L.n + I];
+      const float* rdl = RD + (L.off - stb);
+      for (int I = t; I < L.n; I += T) x[I] = (w * b[I]) * rdl[I];
       __syncthreads();
       for (int I = t; I < L.n; I += T) {
         const int cj = I / L.ny, ci = I - cj * L.ny;
@@ -634,7 +643,7 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
           for (int k = 0; k < 9; k++)
             if (Nb[k] >= 0) acc = acc + st0[q][k] * (x[Nb[k]] + c * xc[Pa[k]]);
           const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
-          out[I] = x1 + w * (b[I] - acc) / st0[q][4];
+          out[I] = x1 + (w * (b[I] - acc)) * rd0[q];
         }
       }
     } else {
@@ -648,7 +657,7 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
         for (int k = 0; k < 9; k++)
           if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (x[Nb[k]] + c * xc[Pa[k]]);
         const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
-        out[I] = x1 + w * (b[I] - acc) / st[4 * L.n + I];
+        out[I] = x1 + (w * (b[I] - acc)) * RD[(L.off - stb) + I];
       }
     }
     __syncthreads();
