@@ -536,20 +536,26 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
     cj0[q] = I / L0.ny;
     ci0[q] = I - cj0[q] * L0.ny;
   }
+  // the dense inverse and the deeper levels' stencils are first read after
+  // level l0's residual: they are stored into LDS there, so their loads
+  // complete while level l0 is smoothed
+  auto store_deeper = [&]() {
 #pragma unroll
-  for (int u = 0; u < (int)(sizeof dv / sizeof dv[0]); u++) {
-    const int e = t + u * T;
-    if (e < nd2) DI[e] = dv[u];
-  }
+    for (int u = 0; u < (int)(sizeof dv / sizeof dv[0]); u++) {
+      const int e = t + u * T;
+      if (e < nd2) DI[e] = dv[u];
+    }
 #pragma unroll
-  for (int u = 0; u < MG_CO_STV; u++) {
-    const int e = t + u * T;
-    if (e < nst) ST[e] = sv[u];
-  }
-  for (int e = t + MG_CO_STV * T; e < nst; e += T) ST[e] = sgs[e];
-  if (t < nrd) RD[t] = rv;
-  for (int e = t + T; e < nrd; e += T) RD[e] = G.rdg[stb + e];
+    for (int u = 0; u < MG_CO_STV; u++) {
+      const int e = t + u * T;
+      if (e < nst) ST[e] = sv[u];
+    }
+    for (int e = t + MG_CO_STV * T; e < nst; e += T) ST[e] = sgs[e];
+    if (t < nrd) RD[t] = rv;
+    for (int e = t + T; e < nrd; e += T) RD[e] = G.rdg[stb + e];
+  };
   if (dn) return;  // the whole workgroup: the flag is per scenario
+  if (l0 == nl) store_deeper();
   __syncthreads();
   // down: x = w b / diag; r = b - A x; r restricted to the next level's b
   for (int l = l0; l < nl; l++) {
@@ -578,6 +584,7 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
           r[I] = b[I] - acc;
         }
       }
+      store_deeper();
     } else {
       const float* rdl = RD + (L.off - stb);
       for (int I = t; I < L.n; I += T) x[I] = (w * b[I]) * rdl[I];
