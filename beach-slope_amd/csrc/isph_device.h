@@ -309,24 +309,28 @@ __device__ __forceinline__ double block_tree(double val, double* lds) {
 }
 
 // Final reduction of nb block partials in the fixed order: lane t adds
-// partials t, t+256, ... sequentially, then the halving tree.
-__device__ __forceinline__ double final_tree(const volatile double* part, int nb, double* lds) {
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < nb; k += RED_BLOCK) acc = acc + part[k];
-  return block_tree(acc, lds);
-}
-
-// "last block finishes": returns true in every thread of the block that
-// observed all other blocks' partials (one counter per reduction site).
-__device__ __forceinline__ bool last_block(unsigned int* counter) {
-  __shared__ bool amlast;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int prev = atomicAdd(counter, 1u);
-    amlast = (prev == gridDim.x - 1);
+// partials t, t+256, ... sequentially, then the halving tree.  p1 (uniform
+// over the block, or null) is a second partial array summed the same way;
+// each lane keeps eight loads of each array in flight.
+__device__ __forceinline__ void final_tree2(const double* __restrict__ p0,
+                                            const double* __restrict__ p1, int nb, double* lds,
+                                            double& s0, double& s1) {
+  double a0 = 0.0, a1 = 0.0;
+  for (int k0 = threadIdx.x; k0 < nb; k0 += 8 * RED_BLOCK) {
+    double v0[8], v1[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int k = k0 + u * RED_BLOCK;
+      v0[u] = (k < nb) ? p0[k] : 0.0;
+      v1[u] = (p1 && k < nb) ? p1[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k0 + u * RED_BLOCK < nb) {
+        a0 = a0 + v0[u];
+        a1 = a1 + v1[u];
+      }
   }
-  __syncthreads();
-  if (amlast) __threadfence();
-  return amlast;
+  s0 = block_tree(a0, lds);
+  s1 = p1 ? block_tree(a1, lds) : 0.0;
 }

@@ -861,10 +861,10 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, doub
   if (op == PFIN_RESID && init == 1 && S->fin) return;
   if (op == PFIN_RESID && init == 2 && !(S->done && !S->fin)) return;
   const int nb = D.nblk;
-  const double s0 = final_tree(D.part, nb, lds);
-  double s1 = 0.0;
-  if (op == PFIN_OMEGA || op == PFIN_BETA || op == PFIN_RESID)
-    s1 = final_tree(D.part + nb, nb, lds);
+  double s0, s1;
+  final_tree2(D.part,
+              (op == PFIN_OMEGA || op == PFIN_BETA || op == PFIN_RESID) ? D.part + nb : nullptr,
+              nb, lds, s0, s1);
   if (threadIdx.x != 0) return;
   const DevParams& P = D.P;
   switch (op) {

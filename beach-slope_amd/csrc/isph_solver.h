@@ -8,7 +8,7 @@
 // Each BiCGStab iteration is four row kernels, three of which write per-block
 // partial dot products (block_tree over 256 rows, in row-block order whatever
 // workgroup computes the block); a one-block finalize kernel then sums the
-// partials in the fixed order (final_tree) and forms alpha, omega or beta on
+// partials in the fixed order (final_tree2) and forms alpha, omega or beta on
 // the device.  The kernel boundary makes the
 // partials visible, so no device-scope fence is needed inside the row
 // kernels.  The host only polls the `done` flag every few iterations.
@@ -167,9 +167,9 @@ __global__ void __launch_bounds__(256) k_finalize(Dev D, int op, int nb, double*
   DevScalars* S = D.sc;
   const bool skip = (op == FIN_ALPHA || op == FIN_OMEGA || op == FIN_BETA) && S->done;
   if (skip) return;
-  const double s0 = final_tree(D.part, nb, lds);
-  double s1 = 0.0;
-  if (op == FIN_OMEGA || op == FIN_BETA) s1 = final_tree(D.part + nb, nb, lds);
+  double s0, s1;
+  final_tree2(D.part, (op == FIN_OMEGA || op == FIN_BETA) ? D.part + nb : nullptr, nb, lds, s0,
+              s1);
   if (threadIdx.x != 0) return;
   const DevParams& P = D.P;
   switch (op) {
