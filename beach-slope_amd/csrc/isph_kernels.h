@@ -149,7 +149,13 @@ __global__ void k_rebin_rank(Dev D) {
   if (bin < 0) return;
   const int s0 = D.cstart[D.cell[i]];
   int rank = 0;
-  for (int j = s0; j < i; j++) rank += (D.pbin[j] == bin);
+  for (int j0 = s0; j0 < i; j0 += 8) {  // eight loads in flight
+    int b[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) b[u] = (j0 + u < i) ? D.pbin[j0 + u] : -1;
+#pragma unroll
+    for (int u = 0; u < 8; u++) rank += (j0 + u < i && b[u] == bin);
+  }
   D.prank[i] = rank;
 }
 

@@ -214,14 +214,25 @@ __device__ __forceinline__ void mg_stencil1_cell(const Dev& D, const MgDev& G, i
   const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
   const int cap = D.cap;
   float* st = G.sten + 9 * L.off;
+  double s[9];
 #pragma unroll
-  for (int q = 0; q < 9; q++) {
-    double s = 0.0;
-    for (int i = i0; i < i1; i++) s = s + G.rowst[q * cap + i];
-    if (q == 4 && i1 == i0) s = 1.0;
-    st[q * L.n + I] = (float)s;
-    if (q == 4) G.rdg[L.off + I] = 1.0f / (float)s;
+  for (int q = 0; q < 9; q++) s[q] = 0.0;
+  for (int b0 = i0; b0 < i1; b0 += 4) {  // four rows' loads in flight, sums in row order
+    double v[4][9];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int q = 0; q < 9; q++) v[u][q] = (b0 + u < i1) ? G.rowst[q * cap + b0 + u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (b0 + u < i1)
+#pragma unroll
+        for (int q = 0; q < 9; q++) s[q] = s[q] + v[u][q];
   }
+  if (i1 == i0) s[4] = 1.0;
+#pragma unroll
+  for (int q = 0; q < 9; q++) st[q * L.n + I] = (float)s[q];
+  G.rdg[L.off + I] = 1.0f / (float)s[4];
 }
 
 // level l+1 cell I from level l: sum of the 2x2 children's stencils
