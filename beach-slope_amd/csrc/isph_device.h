@@ -64,6 +64,10 @@ struct Dev {
   DevScalars* sc;
   int cap;
   int nblk;  // row blocks of RED_BLOCK rows covering the host's upper bound of n
+  // per row block: running maximum of |velocity| (bits of a non-negative double)
+  // at [b] and of |nu_t| at [nmaxb + b], folded into the scalars by timestep
+  unsigned long long* maxb;
+  int nmaxb;
   // gathered particle state (ping-pong buffers swapped by the host)
   double *x, *y, *u, *v, *us, *vs, *xs, *ys, *p, *m, *rho, *rho0, *vol0, *c;
   uint8_t *tid, *flags, *vic;
@@ -306,6 +310,43 @@ __device__ __forceinline__ double block_tree(double val, double* lds) {
   }
   __syncthreads();
   return v;  // valid in thread 0
+}
+
+// maxima over a block of two non-negative double bit patterns (valid in
+// thread 0); every thread of the block must call it
+__device__ __forceinline__ void block_max2_u64(unsigned long long& a, unsigned long long& b) {
+  __shared__ unsigned long long wa[RED_BLOCK / 64], wb[RED_BLOCK / 64];
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) {
+    const unsigned long long oa = __shfl_xor(a, s, 64), ob = __shfl_xor(b, s, 64);
+    a = (oa > a) ? oa : a;
+    b = (ob > b) ? ob : b;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wa[threadIdx.x >> 6] = a;
+    wb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      a = (wa[w] > a) ? wa[w] : a;
+      b = (wb[w] > b) ? wb[w] : b;
+    }
+  }
+  __syncthreads();
+}
+
+// sum over a block (valid in thread 0); every thread of the block must call it
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v) {
+  __shared__ unsigned long long ws[RED_BLOCK / 64];
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) v += ws[w];
+  __syncthreads();
+  return v;
 }
 
 // Final reduction of nb block partials in the fixed order: lane t adds

@@ -550,9 +550,7 @@ __global__ void __launch_bounds__(256) k_int_vel(Dev D) {
 // Raw values are written first, then divided by the diagonal (Jacobi,
 // isph.f90:432-440) once the diagonal is known.
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
-  PIDX();
-  if (i >= n) return;
+__device__ __forceinline__ int ppe_assemble_row(const Dev& D, int i) {
   const DevParams& P = D.P;
   const int cap = D.cap, cnt = D.ncnt[i];
   const double dt = D.sc->dt;
@@ -677,19 +675,43 @@ __global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
   }
   D.alen[i] = L;
   D.b[i] = rhs / t1;
-  atomicAdd(&D.sc->nnz, (unsigned long long)L);
+  return L;
+}
+
+// the stored entries are counted per workgroup: one atomic per workgroup
+// rather than one per row on a single address
+__global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
+  PIDX();
+  const int L = (i < n) ? ppe_assemble_row(D, i) : 0;
+  const unsigned long long s = block_sum_u64((unsigned long long)L);
+  if (threadIdx.x == 0 && s != 0) atomicAdd(&D.sc->nnz, s);
 }
 
 // ---------------------------------------------------------------------
 // comp_vel, integrator.f90:705-843 (with the running maxima of timestep)
 // ---------------------------------------------------------------------
-__device__ __forceinline__ void atomic_max_pos(unsigned long long* a, double v) {
-  if (v == v) atomicMax(a, (unsigned long long)__double_as_longlong(v));
+// bits of |v| for the running maxima (0 for NaN, which they leave out)
+__device__ __forceinline__ unsigned long long max_bits(double v) {
+  const double a = fabs(v);
+  return (a == a) ? (unsigned long long)__double_as_longlong(a) : 0ull;
 }
 
-__global__ void __launch_bounds__(256) k_comp_vel(Dev D) {
-  PIDX();
-  if (i >= n) return;
+// the row block's maxima added to its running maxima (the block owns its
+// slot, so no atomic is needed); every thread of the block must call it
+__device__ __forceinline__ void block_running_max(const Dev& D, unsigned long long vb,
+                                                  unsigned long long nb) {
+  block_max2_u64(vb, nb);
+  if (threadIdx.x == 0) {
+    unsigned long long* sv = D.maxb + blockIdx.x;
+    unsigned long long* sn = D.maxb + D.nmaxb + blockIdx.x;
+    const unsigned long long ov = *sv, on = *sn;
+    *sv = (vb > ov) ? vb : ov;
+    *sn = (nb > on) ? nb : on;
+  }
+}
+
+__device__ __forceinline__ void comp_vel_row(const Dev& D, int i, unsigned long long& vbits,
+                                             unsigned long long& nbits) {
   const DevParams& P = D.P;
   const double dt = D.sc->dt;
   const bool buf = (D.flags[i] & F_BUFFER) != 0;
@@ -742,13 +764,22 @@ __global__ void __launch_bounds__(256) k_comp_vel(Dev D) {
     D.v[i] = v;
     const double vel = sqrt((u * u) + (v * v)) / ni;
     D.vel[i] = vel;
-    atomic_max_pos(&D.sc->umax_bits, fabs(vel));
+    vbits = max_bits(vel);
   }
   if (buf) {
     D.u[i] = D.us[i];
     D.v[i] = 0.0;
   }
-  atomic_max_pos(&D.sc->numax_bits, fabs(D.nut[i]));
+  nbits = max_bits(D.nut[i]);
+}
+
+// the running maxima of |velocity| and |nu_t| go to per-block slots (one
+// atomic per particle on a single address took half of this kernel's time)
+__global__ void __launch_bounds__(256) k_comp_vel(Dev D) {
+  PIDX();
+  unsigned long long vbits = 0, nbits = 0;
+  if (i < n) comp_vel_row(D, i, vbits, nbits);
+  block_running_max(D, vbits, nbits);
 }
 
 // comp_pos, integrator.f90:845-883
@@ -853,12 +884,37 @@ __global__ void k_densityupdate(Dev D) {
   D.rho[i] = r0 * D.por[i];
 }
 
-// timestep, integrator.f90:885-953 (one thread)
+// timestep, integrator.f90:885-953 (one workgroup: the row blocks' running
+// maxima are folded into the scalars and cleared, then one thread)
 __device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
-__global__ void k_timestep(Dev D) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ void __launch_bounds__(256) k_timestep(Dev D) {
+  if (blockIdx.x != 0) return;
+  unsigned long long mv = 0, mn = 0;
+  for (int b0 = threadIdx.x; b0 < D.nmaxb; b0 += 4 * blockDim.x) {
+    unsigned long long qv[4], qn[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int b = b0 + u * (int)blockDim.x;
+      qv[u] = (b < D.nmaxb) ? D.maxb[b] : 0ull;
+      qn[u] = (b < D.nmaxb) ? D.maxb[D.nmaxb + b] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int b = b0 + u * (int)blockDim.x;
+      mv = (qv[u] > mv) ? qv[u] : mv;
+      mn = (qn[u] > mn) ? qn[u] : mn;
+      if (b < D.nmaxb) {
+        D.maxb[b] = 0ull;
+        D.maxb[D.nmaxb + b] = 0ull;
+      }
+    }
+  }
+  block_max2_u64(mv, mn);
+  if (threadIdx.x != 0) return;
   const DevParams& P = D.P;
   DevScalars* S = D.sc;
+  if (mv > S->umax_bits) S->umax_bits = mv;
+  if (mn > S->numax_bits) S->numax_bits = mn;
   S->umax = __longlong_as_double((long long)S->umax_bits);
   S->numax = __longlong_as_double((long long)S->numax_bits);
   const double dl = P.dl, sig1 = P.sig1;

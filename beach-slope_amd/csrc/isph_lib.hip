@@ -168,7 +168,7 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   // vectors, level-0 multigrid vectors, setup rows
   if (precision) per += 2 * 4 * ((ROW_CAP + 1) / 2) + 5 * 4 + 2 * 4 + 9 * 8;
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
-  b += 8 * 2 * (n_cap / 256 + 2);               // reduction partials
+  b += 2 * 8 * 2 * (n_cap / 256 + 2);           // reduction partials, running maxima
   // cell levels: stencil, x, b, r and 1/diagonal per unknown, the levels
   // holding 4/3 of the cell count
   if (precision) b += 4 * 13 * ((4 * ncell) / 3 + MG_MAX_LEVELS) + 8 * MG_DENSE_MAX * MG_DENSE_MAX;
@@ -234,6 +234,10 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   rc |= dalloc(c, &D.srho, c->scap);
   rc |= dalloc(c, &D.sstart, c->ncell + 1);
   rc |= dalloc(c, &D.part, 2 * (N / 256 + 2));
+  D.nmaxb = (int)(N / 256 + 2);
+  rc |= dalloc(c, &D.maxb, 2 * (int64_t)D.nmaxb);
+  if (rc == 0 && hipMemset(D.maxb, 0, sizeof(unsigned long long) * 2 * (size_t)D.nmaxb) != hipSuccess)
+    rc |= ISPH_EHIP;  // the running maxima start at zero
   rc |= dalloc(c, &D.sc, 1);
   rc |= dalloc(c, &c->d_bn2, 1);
   if (params->precision) {
@@ -1240,7 +1244,7 @@ __global__ void k_reset_max(Dev D) {
 
 extern "C" int isph_timestep(isph_ctx* c) {
   if (!c) return ISPH_EARG;
-  hipLaunchKernelGGL(k_timestep, dim3(1), dim3(64), 0, c->stream, c->D);
+  hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, c->stream, c->D);
   hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(64), 0, c->stream, c->D);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -1258,10 +1262,10 @@ extern "C" int isph_scalarupdate(isph_ctx* c, double stime) {
   return 0;
 }
 
-__global__ void k_numax_init(Dev D) {
+__global__ void __launch_bounds__(256) k_numax_init(Dev D) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) D.sc->umax_bits = 0ull;
-  if (i < D.sc->n) atomic_max_pos(&D.sc->numax_bits, fabs(D.nut[i]));
+  block_running_max(D, 0ull, (i < D.sc->n) ? max_bits(D.nut[i]) : 0ull);
 }
 
 extern "C" int isph_init_state(isph_ctx* c) {

@@ -183,6 +183,31 @@ def test_fused_steps_bit_exact_dam_break(pkg, oracle, cases):
     compare_scalars(g, o, "after 5 fused steps")
 
 
+def test_fused_steps_adaptive_dt_bit_exact(pkg, oracle, cases):
+    """The adaptive time step (fixed_dt 0, integrator.f90:927-937) takes dt
+    from the running maxima of |velocity| and |nu_t| that comp_vel leaves for
+    timestep: 3 fused steps of the moving tiny box, dt and every field bit
+    for bit, with dt changing between steps."""
+    c = cases.tiny_box(nx=12, ny=10, solids=True, patch=True)
+    prm = dict(c.params, fixed_dt=0)
+    g = pkg.Scenario(c, params=prm)
+    o = oracle.Oracle(c, params=prm)
+    g.init_state()
+    assert o.call("init_state") == 0
+    compare_scalars(g, o, "init")
+    dts = []
+    for k in range(3):
+        _, st = g.step(1)
+        rc, sto = o.step(1)
+        assert rc == 0
+        assert st.iterations == sto.iterations
+        compare_scalars(g, o, f"adaptive step {k}")
+        dts.append(g.scalars().dt)
+    compare_state(g, o, "after 3 adaptive steps")
+    print(f"adaptive dt over 3 steps: {dts}", flush=True)
+    assert len(set(dts)) > 1
+
+
 def test_fp64_breakdown_restart_bit_exact(pkg, oracle, cases):
     """The jittered dam break's FP64 solve breaks down at step 24 (rho or
     omega vanishes at a relative residual near 1e-8, above the 1e-10 target);
