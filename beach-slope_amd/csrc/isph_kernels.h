@@ -92,9 +92,7 @@ __device__ __forceinline__ int dest_offset_cj(int cls) {
   }
 }
 
-__global__ void k_rebin_classify(Dev D, int alt) {
-  PIDX();
-  if (i >= n) return;
+__device__ __forceinline__ int rebin_bin(const Dev& D, int i, int alt) {
   const DevParams& P = D.P;
   int cls;
   const int key = D.cell[i];
@@ -130,14 +128,26 @@ __global__ void k_rebin_classify(Dev D, int alt) {
       if (dci < 1 || dci > P.celly - 2 || dcj < 1 || dcj > P.cellx - 2) cls = CLS_DROP;
     }
   }
-  int bin = -1;
-  if (cls != CLS_DROP) {
-    const int ci = key % P.celly, cj = key / P.celly;
-    const int dkey = (cj + dest_offset_cj(cls)) * P.celly + (ci + dest_offset_ci(cls));
-    bin = dkey * NCLS + cls;
-    atomicAdd(&D.bincnt[bin], 1);
-  }
-  D.pbin[i] = bin;
+  if (cls == CLS_DROP) return -1;
+  const int ci = key % P.celly, cj = key / P.celly;
+  const int dkey = (cj + dest_offset_cj(cls)) * P.celly + (ci + dest_offset_ci(cls));
+  return dkey * NCLS + cls;
+}
+
+// The lanes of a wavefront hold consecutive particles, so a run of lanes with
+// one bin (a cell's stayers, mostly) adds its length with one atomic from its
+// first lane.  The counts are the same integers as one atomic per particle.
+__global__ void k_rebin_classify(Dev D, int alt) {
+  PIDX();
+  const int bin = (i < n) ? rebin_bin(D, i, alt) : -2;
+  if (i < n) D.pbin[i] = bin;
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(bin, 1, 64);
+  const bool head = (lane == 0) || (prev != bin);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long later = (lane == 63) ? 0ull : (heads >> (lane + 1));
+  const int len = later ? __ffsll((long long)later) : 64 - lane;
+  if (head && bin >= 0) atomicAdd(&D.bincnt[bin], len);
 }
 
 // rank of a particle among the earlier particles of its source cell with the
