@@ -2,7 +2,8 @@
 on one MI355X.
 
 A step is one production-loop iteration of the reference (main.f90:199-261)
-over the 650k-particle flume: scalar transport, projection, rebin, neighbour
+over the 650k-particle flume with its piston wavemaker: scalar transport,
+projection, the paddle's motion, rebin, neighbour
 search, porosity, SPS, intermediate velocity, PPE assembly and solve,
 pressure-gradient update, trapezoid positions, second rebin and neighbour
 search, free surface, porosity, shifting, density, time step, SPS.  State is
@@ -365,7 +366,8 @@ def main() -> int:
         "dtype": "f32+f64" if "chosen" in res else "f64",
         "data": "synthetic",
         "config": {"workload": "flume_650k: 10 m x 0.4 m flume, dx 0.0025, 4000x160 fluid "
-                               "+ 3 wall layers, scalar patch, dt 5e-5",
+                               "+ 3 wall layers, piston wavemaker (H 0.05 m, T 1.5 s), "
+                               "scalar patch, dt 5e-5",
                    "particles": head["n"], "ppe_rel_tol": args.tol,
                    "form": "chosen" if "chosen" in res else "plain"},
         "roofline": {"bound": "hbm", "achieved": head["kernel_gbs"], "peak": HBM_PEAK_GBS,
