@@ -17,6 +17,11 @@ Two forms are measured on the same input:
 Both solve to the same relative residual (1e-8) so their throughputs compare
 like for like.  `value` is the chosen form.
 
+Each form also reports the whole step against the HBM roofline: its
+algorithmic bytes per particle-step (bytes_model.step_bytes, at the measured
+iteration count), the rate that gives at the measured speed, and the
+fraction of the 8 TB/s peak.
+
 The roofline object is for the dominant kernel, the solver's SpMV t = A s
 with its two fused dot products, timed with HIP events on the context's
 stream over the timed region.  Its `traffic` (HBM bytes per launch) comes
@@ -90,12 +95,20 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
     from beach_slope_amd import bytes_model
     bytes_per = bytes_model.kernel_bytes(KERNEL_OF[kname], n, nnz)
     achieved = bytes_per / (avg_us * 1e-6) / 1e9 if launches else 0.0
+    # the whole step against the HBM roofline: algorithmic bytes per step
+    # (bytes_model.step_bytes) at the measured rate, over the 8 TB/s peak
+    step_b = bytes_model.step_bytes("precision" if precision else "plain", n, nnz,
+                                    float(np.mean(iters)))
+    step_gbs = step_b * steps / wall / 1e9
     return dict(
         n=n, seconds=wall, ms_per_step=1e3 * wall / steps,
         particle_steps_per_s=n * steps / wall, iters_mean=float(np.mean(iters)),
         outer_mean=float(np.mean(outer)), nnz_per_row=nnz / n,
         kernel=kname, kernel_avg_us=avg_us, kernel_launches=launches,
         kernel_bytes=bytes_per, kernel_gbs=achieved,
+        step_bytes_per_particle=step_b / n, step_gbs=step_gbs,
+        step_hbm_frac=step_gbs / HBM_PEAK_GBS,
+        hbm_bound_particle_steps_per_s=n * HBM_PEAK_GBS * 1e9 / step_b,
     )
 
 
