@@ -1235,17 +1235,22 @@ extern "C" int isph_densityupdate(isph_ctx* c) {
   return 0;
 }
 
-__global__ void k_reset_max(Dev D) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+// clears the running maxima of timestep; with `slots`, also the row blocks'
+// pending maxima (timestep's fold clears those itself)
+__global__ void k_reset_max(Dev D, int slots) {
+  if (blockIdx.x != 0) return;
+  if (threadIdx.x == 0) {
     D.sc->umax_bits = 0ull;
     D.sc->numax_bits = 0ull;
   }
+  if (slots)
+    for (int b = threadIdx.x; b < 2 * D.nmaxb; b += blockDim.x) D.maxb[b] = 0ull;
 }
 
 extern "C" int isph_timestep(isph_ctx* c) {
   if (!c) return ISPH_EARG;
   hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, c->stream, c->D);
-  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(64), 0, c->stream, c->D);
+  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(64), 0, c->stream, c->D, 0);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -1274,7 +1279,7 @@ extern "C" int isph_init_state(isph_ctx* c) {
   if (rc) return rc;
   LAUNCH(k_effpor, c->n_hi, c->D);
   LAUNCH(k_eddyvis, c->n_hi, c->D);
-  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(64), 0, c->stream, c->D);
+  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(256), 0, c->stream, c->D, 1);
   LAUNCH(k_numax_init, c->n_hi, c->D);
   rc = isph_timestep(c);
   if (rc) return rc;
