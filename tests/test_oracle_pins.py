@@ -87,20 +87,23 @@ def test_reduction_tree_is_exact_on_integers_and_close_on_reals(L):
 
 # ---- solver: the reference's own known answer ----------------------------
 def test_solver_known_answer_5x5(oracle, cases):
-    # solvertest.f90:10-16: row=(1,4,6,8,11,14), col=(1,3,4,2,4,1,3,2,3,4,1,2,5),
-    # all values 1, b=(8,6,4,9,8); exact solution x = 1..5
+    # solvertest.f90:10-16, kept as data in tests/golden/solvertest_5x5.json:
+    # a 1-based CSR system with exact solution x = 1..5
     import ctypes
-    rowp = [0, 3, 5, 7, 10, 13]
-    col = [0, 2, 3, 1, 3, 0, 2, 1, 2, 3, 0, 1, 4]
-    n, W = 5, oracle.RCAP
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "solvertest_5x5.json")) as f:
+        g = json.load(f)
+    rowp = [r - 1 for r in g["row"]]
+    col = [c - 1 for c in g["col"]]
+    n, W = len(rowp) - 1, oracle.RCAP
     ln = np.array([rowp[i + 1] - rowp[i] for i in range(n)], np.int32)
     C = np.zeros((n, W), np.int32)
     V = np.zeros((n, W))
     for i in range(n):
-        cs = col[rowp[i]:rowp[i + 1]]
-        C[i, :len(cs)] = cs
-        V[i, :len(cs)] = 1.0
-    b = np.array([8.0, 6.0, 4.0, 9.0, 8.0])
+        C[i, :ln[i]] = col[rowp[i]:rowp[i + 1]]
+        V[i, :ln[i]] = g["val"][rowp[i]:rowp[i + 1]]
+    b = np.array(g["b"], dtype=np.float64)
     x = np.zeros(n)
     c = cases.tiny_box()
     p = dict(c.params)
@@ -110,7 +113,7 @@ def test_solver_known_answer_5x5(oracle, cases):
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     rc = oracle.lib().or_solve_system(o.s, n, P(ln), P(C), P(V), P(b), P(x), ctypes.byref(st))
     assert rc == 0
-    assert np.allclose(x, [1, 2, 3, 4, 5], atol=1e-10)
+    assert np.allclose(x, g["x"], atol=1e-10)
 
 
 # ---- pipeline phases on a small jittered box ------------------------------
