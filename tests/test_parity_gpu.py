@@ -295,3 +295,42 @@ def test_precision_int32_column_path_matches_offsets(pkg, cases):
     assert st_a.converged and st_b.converged
     assert (st_a.iterations, st_a.outer) == (st_b.iterations, st_b.outer)
     assert np.array_equal(xa, xb)
+
+
+def test_solver_known_answer_5x5(pkg, oracle, cases):
+    """The FP64 solve on the reference's own known answer (solvertest.f90:10-16,
+    tests/golden/solvertest_5x5.json): x = 1..5, bit-identical to the oracle's
+    BiCGStab on the same rows, iterations included."""
+    import ctypes
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "solvertest_5x5.json")) as f:
+        g = json.load(f)
+    rowp = [r - 1 for r in g["row"]]
+    n, W = len(rowp) - 1, oracle.RCAP
+    ln = np.array([rowp[i + 1] - rowp[i] for i in range(n)], np.int32)
+    col = np.zeros((n, W), np.int32)
+    val = np.zeros((n, W))
+    for i in range(n):
+        col[i, :ln[i]] = [c - 1 for c in g["col"][rowp[i]:rowp[i + 1]]]
+        val[i, :ln[i]] = g["val"][rowp[i]:rowp[i + 1]]
+    b = np.array(g["b"], dtype=np.float64)
+    c = cases.tiny_box()
+    prm = dict(c.params)
+    prm.update(precision=0, tol=1e-14, max_iter=200)
+    dev = pkg.Scenario(c, params=prm)
+    dev.init_state()
+    dev.load_system(ln, col, val, b, x0=np.zeros(n))
+    st = dev.ppe_solve()
+    x = dev.solution()
+    o = oracle.Oracle(c, params=prm)
+    ox = np.zeros(n)
+    ost = oracle.SolveStats()
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert oracle.lib().or_solve_system(o.s, n, P(ln), P(col), P(val), P(b), P(ox),
+                                        ctypes.byref(ost)) == 0
+    print(f"5x5: {st.iterations} iterations, x = {x}", flush=True)
+    assert np.allclose(x, g["x"], atol=1e-10)
+    assert st.iterations == ost.iterations
+    assert_same("5x5 solution", x, ox)
+    dev.close()
