@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
   // the candidates are tested and the list written first, then the list is
   // walked in its order.  Each pass requests a batch of positions (NB_CAND
   // candidates, NB_PAIR neighbours) before testing or using any of them.  A
-  // lane walks ~100 candidates of four cells; batches of 8 candidates took
+  // lane walks ~66 candidates of four cells; batches of 8 candidates took
   // 8% off the kernel, and larger batches of either kind gained nothing
   // more while costing registers.  r is recomputed from the same operands,
   // so every value and sum is the one of a single pass.
