@@ -45,6 +45,15 @@
 
 #include "../include/isph_abi.h"
 
+/* Planted mistakes for the pin tests (tests/test_oracle_pins.py): a build
+ * with -DOR_MUTANT=k changes the one expression marked MUT(k) below, so a pin
+ * can show that it fails on a plausible transcription error.  The default
+ * build (OR_MUTANT = 0) has none of them. */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+#define MUT(k) (OR_MUTANT == (k))
+
 #define KCAP ISPH_NBR_CAP
 #define RCAP (ISPH_NBR_CAP + 2)
 #define RED_BLOCK 256
@@ -379,9 +388,11 @@ void or_projection(ostate *s) {
   for (int64_t i = 0; i < s->n; i++) {
     if (s->tid[i] == ISPH_TID_FLUID) {
       s->xs[i] = s->x[i];
-      s->x[i] = s->x[i] + ((dt * s->u[i]) / s->por[i]);
+      s->x[i] = MUT(14) ? s->x[i] + ((dt * s->u[i]) * s->por[i])
+                        : s->x[i] + ((dt * s->u[i]) / s->por[i]);
       s->ys[i] = s->y[i];
-      s->y[i] = s->y[i] + ((dt * s->v[i]) / s->por[i]);
+      s->y[i] = MUT(14) ? s->y[i] + ((dt * s->v[i]) * s->por[i])
+                        : s->y[i] + ((dt * s->v[i]) / s->por[i]);
     }
   }
 }
@@ -563,7 +574,7 @@ void or_effpor(ostate *s) {
     double x = s->x[i], y = s->y[i];
     int di[4] = {0, 0, 0, 0}, dj[4] = {0, 0, 0, 0};
     if (x <= xmid && y <= ymid) {         /* porous.f90:138-140 */
-      di[1] = 0; dj[1] = -1; di[2] = 1; dj[2] = 0; di[3] = 1; dj[3] = -1;
+      di[1] = 0; dj[1] = MUT(5) ? 1 : -1; di[2] = 1; dj[2] = 0; di[3] = 1; dj[3] = MUT(5) ? 1 : -1;
     } else if (x > xmid && y <= ymid) {   /* porous.f90:147-149 */
       di[1] = 0; dj[1] = 1; di[2] = 1; dj[2] = 0; di[3] = 1; dj[3] = 1;
     } else if (x > xmid && y > ymid) {    /* porous.f90:156-158 */
@@ -588,7 +599,7 @@ void or_effpor(ostate *s) {
     double sp = sqrt((s->u[i] * s->u[i]) + (s->v[i] * s->v[i]));
     double a = -((((o_vis(s->rho0[i]) / s->rho0[i]) * lamk) * ((1.0 - n) * (1.0 - n))) /
                  (n3 * (d50 * d50)));
-    double b = (((fch * sp) * sqrt(lamk)) * (1.0 - n)) / (sqrt(n3) * d50);
+    double b = (((fch * sp) * sqrt(lamk)) * (1.0 - n)) / (sqrt(MUT(6) ? n * n : n3) * d50);
     s->resist[i] = a - b;
   }
 }
@@ -638,7 +649,7 @@ void or_int_vel(ostate *s) {
       t1 = t1 + (((s->u[j] / s->por[j]) - (s->u[i] / s->por[i])) * gx) * (s->m[j] / s->rho[j]);
       t2 = t2 + (((s->v[j] / s->por[j]) - (s->v[i] / s->por[i])) * gy) * (s->m[j] / s->rho[j]);
     }
-    s->varts[i] = (((1.0 * (dl * dl)) * sqrt((t1 * t1) + (t2 * t2))) *
+    s->varts[i] = (((1.0 * (dl * dl)) * sqrt((t1 * t1) + (MUT(17) ? 0.0 : t2 * t2))) *
                    sqrt((s->u[i] * s->u[i]) + (s->v[i] * s->v[i]))) /
                   ((s->umax + eps6) * s->por[i]);
   }
@@ -672,14 +683,14 @@ void or_int_vel(ostate *s) {
         double pvol = s->m[j] / s->rho[j];
         double pd = 1.0 / ((r * r) + lam);
         double dxij = s->x[i] - s->x[j], dyij = s->y[i] - s->y[j];
-        double a1 = ((((2 * pvol) * (s->u[i] - s->u[j])) * dxij) * gx) * pd;
-        double a2 = ((((2 * pvol) * (s->u[i] - s->u[j])) * dyij) * gy) * pd;
+        double a1 = (((((MUT(7) ? 1 : 2) * pvol) * (s->u[i] - s->u[j])) * dxij) * gx) * pd;
+        double a2 = (((((MUT(7) ? 1 : 2) * pvol) * (s->u[i] - s->u[j])) * dyij) * gy) * pd;
         us = us + (a1 + a2) * (cmu + vart);
         if (s->tid[j] > 2) {
           if ((s->nut[i] < eps6) && (s->nut[j] < eps6))
             us = us + ((a1 + a2) * (s->nut[i] + s->nut[j])) * 0.50;
           else
-            us = us + (a1 + a2) * (((2 * s->nut[i]) * s->nut[j]) / (s->nut[i] + s->nut[j]));
+            us = us + (a1 + a2) * ((((MUT(19) ? 1 : 2) * s->nut[i]) * s->nut[j]) / (s->nut[i] + s->nut[j]));
         }
         a1 = ((((2 * pvol) * (s->v[i] - s->v[j])) * dxij) * gx) * pd;
         a2 = ((((2 * pvol) * (s->v[i] - s->v[j])) * dyij) * gy) * pd;
@@ -691,11 +702,11 @@ void or_int_vel(ostate *s) {
             vs = vs + (a1 + a2) * (((2 * s->nut[i]) * s->nut[j]) / (s->nut[i] + s->nut[j]));
         }
         double tk = (pvol * (s->tke[j] - s->tke[i])) * gx;
-        us = us - ((tk * ni) * 2.0) / 3;
+        us = MUT(8) ? us + ((tk * ni) * 2.0) / 3 : us - ((tk * ni) * 2.0) / 3;
         tk = (pvol * (s->tke[j] - s->tke[i])) * gy;
         vs = vs - ((tk * ni) * 2.0) / 3;
       }
-      double den = 1.0 - ((dt * s->resist[i]) * ni);
+      double den = MUT(9) ? 1.0 / (1.0 + ((dt * s->resist[i]) * ni)) : 1.0 - ((dt * s->resist[i]) * ni);
       us = (s->u[i] + (dt * us)) / den;
       vs = (s->v[i] + (dt * vs)) / den;
       vs = vs + ((g * ni) * dt);
@@ -753,7 +764,8 @@ int or_ppe_assemble(ostate *s) {
         val[ins] = -(t1 + t2); col[ins] = j; wr[ins] = 1;
         diag = diag + (t1 + t2) * lamp;
         if (s->tid[j] == ISPH_TID_FLUID) {
-          t1 = (pvol * (((s->us[j] * rho_ij) / nj) - ((s->us[i] * rho_ij) / ni))) * gx;
+          t1 = MUT(10) ? (pvol * (((s->us[i] * rho_ij) / ni) - ((s->us[j] * rho_ij) / nj))) * gx
+                       : (pvol * (((s->us[j] * rho_ij) / nj) - ((s->us[i] * rho_ij) / ni))) * (MUT(11) ? gy : gx);
           t2 = (pvol * (((s->vs[j] * rho_ij) / nj) - ((s->vs[i] * rho_ij) / ni))) * gy;
         } else {
           t1 = (pvol * (((s->us[j] * rho_i) / ni) - ((s->us[i] * rho_i) / ni))) * gx;
@@ -763,7 +775,7 @@ int or_ppe_assemble(ostate *s) {
       } else if (tidi == ISPH_TID_GHOST) {
         t1 = (((2 * pvol) * z1) * dx) * pd;
         t2 = (((2 * pvol) * z2) * dy) * pd;
-        if (s->tid[j] <= 2) {
+        if (s->tid[j] <= 2 || MUT(18)) {
           val[ins] = -(t1 + t2); col[ins] = j; wr[ins] = 1;
           diag = diag + (t1 + t2);
         }
@@ -966,9 +978,10 @@ void or_comp_vel(ostate *s) {
         if (s->tid[j] == ISPH_TID_FLUID) {
           double a = ((pj * rhoi) / ni) + ((pi * rhoj) / nj);
           double den = rhoi * ((rhoi / ni) + (rhoj / nj));
-          t1 = (2 * (((V * (ni * ni)) * a) * gx)) / den;
+          const double nn = MUT(2) ? ni : (ni * ni);
+          t1 = ((MUT(1) ? 1 : 2) * (((V * nn) * a) * gx)) / den;
           u = u - t1 * dt;
-          t1 = (2 * (((V * (ni * ni)) * a) * gy)) / den;
+          t1 = ((MUT(1) ? 1 : 2) * (((V * nn) * a) * gy)) / den;
           v = v - t1 * dt;
         } else {
           t1 = (((V * (ni * ni)) * (pj + pi)) * gx) / rhoi;
@@ -998,7 +1011,7 @@ void or_comp_pos(ostate *s) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < s->n; i++) {
     if (s->tid[i] != ISPH_TID_FLUID) continue;
-    s->x[i] = ((s->x[i] + s->xs[i]) + ((dt * s->u[i]) / s->por[i])) / 2.0;
+    s->x[i] = ((s->x[i] + (MUT(15) ? s->x[i] : s->xs[i])) + ((dt * s->u[i]) / s->por[i])) / 2.0;
     s->y[i] = ((s->y[i] + s->ys[i]) + ((dt * s->v[i]) / s->por[i])) / 2.0;
     if ((s->flags[i] & ISPH_F_BUFFER) && (s->x[i] > s->P.buf_xmin) && (s->x[i] < s->P.buf_xmax))
       s->flags[i] &= (uint8_t)~ISPH_F_BUFFER;
@@ -1011,7 +1024,8 @@ void or_freesurf(ostate *s) {
   for (int64_t i = 0; i < s->n; i++) {
     if (s->tid[i] == ISPH_TID_FLUID && !(s->flags[i] & ISPH_F_BUFFER) &&
         s->gradvx[i] <= s->P.fs_thresh) {
-      for (int m = 0; m < s->ncnt[i]; m++) s->flags[s->nbr[i * KCAP + m]] |= ISPH_F_VICINITY;
+      for (int m = 0; m < s->ncnt[i]; m++)
+        s->flags[MUT(16) ? i : s->nbr[i * KCAP + m]] |= ISPH_F_VICINITY;
     }
   }
 }
@@ -1039,8 +1053,9 @@ void or_opt2_shift(ostate *s) {
       double z1 = s->klt1[i * KCAP + m], z2 = s->klt2[i * KCAP + m];
       double wq = o_wab(s->nr[i * KCAP + m], h) / wdl;
       double f = 1.0 + (0.20 * pint) * ((wq * wq) * (wq * wq));
-      xsa = xsa - (((((csh * (h * h)) * frac) * (((1 - normx * normx) * z1) - ((normx * normy) * z2))) *
-                   (s->m[j] / s->rho[j])) * f);
+      const double dxs = (((((csh * (h * h)) * frac) * (((1 - normx * normx) * z1) - ((normx * normy) * z2))) *
+                           (s->m[j] / s->rho[j])) * f);
+      xsa = MUT(3) ? xsa + dxs : xsa - dxs;
     }
     for (int m = 0; m < cnt; m++) {
       int32_t j = s->nbr[i * KCAP + m];
@@ -1061,7 +1076,7 @@ void or_opt2_shift(ostate *s) {
       t1 = t1 + (gx * (s->u[j] - s->u[i])) * V;
       t2 = t2 + (gy * (s->u[j] - s->u[i])) * V;
     }
-    s->us[i] = t1 * xsa + t2 * ysa;
+    s->us[i] = MUT(4) ? t2 * xsa + t1 * ysa : t1 * xsa + t2 * ysa;
     t1 = 0.0;
     t2 = 0.0;
     for (int m = 0; m < cnt; m++) {
@@ -1173,11 +1188,12 @@ void or_scalart(ostate *s) {
       } else {
         Dxxi = Dm + s->nut[i] / tsc;
         Dxxj = Dm + s->nut[j] / tsc;
-        if ((s->nut[i] < eps6) && (s->nut[j] < eps6)) t1x = (Dxxi + Dxxj) * 0.50;
+        if ((s->nut[i] < eps6) && (s->nut[j] < eps6)) t1x = (Dxxi + Dxxj) * (MUT(12) ? 1.0 : 0.50);
         else t1x = ((2 * Dxxi) * Dxxj) / (Dxxi + Dxxj);
       }
       double t1y = t1x, t2 = 1.0;
       double t3 = (s->vol0[i] * (ni + nj)) / (ni * nj);
+      if (MUT(13)) t3 = -t3;
       double dx = s->x[i] - s->x[j], dy = s->y[i] - s->y[j];
       double hwx = o_wabd(dx, r, h), hwy = o_wabd(dy, r, h);
       double den = (r * r) + lam;

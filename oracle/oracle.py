@@ -55,15 +55,16 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
-_lib = None
+_libs: dict = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
+def lib(path=None):
+    """The oracle library (the default build, or the one at `path`)."""
+    path = str(path or LIB_PATH)
+    if path not in _libs:
+        if path == str(LIB_PATH) and not LIB_PATH.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
+        L = C.CDLL(path)
         vp, i64, d = C.c_void_p, C.c_int64, C.c_double
         L.or_create.restype = vp
         L.or_create.argtypes = [C.POINTER(IsphParams), i64, i64]
@@ -110,8 +111,8 @@ def lib():
         L.or_wavemaker_motion.argtypes = [d, d, d, d, d, C.POINTER(d), C.POINTER(d)]
         L.or_set_wavemaker.argtypes = [vp, d, d, d, d]
         L.or_wavemaker.argtypes = [vp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 _DTYPES = {
@@ -131,8 +132,9 @@ def _ptr(a: np.ndarray):
 class Oracle:
     """One scenario held by the CPU oracle."""
 
-    def __init__(self, case, cap: int | None = None, params: dict | None = None):
-        L = lib()
+    def __init__(self, case, cap: int | None = None, params: dict | None = None,
+                 library=None):
+        self.L = L = library if library is not None else lib()  # noqa
         self.case = case
         self.pdict = dict(case.params if params is None else params)
         self.P = make_params(self.pdict)
@@ -163,20 +165,20 @@ class Oracle:
     def __del__(self):
         try:
             if getattr(self, "s", None):
-                lib().or_destroy(self.s)
+                self.L.or_destroy(self.s)
                 self.s = None
         except Exception:
             pass
 
     @property
     def n(self) -> int:
-        return int(lib().or_n(self.s))
+        return int(self.L.or_n(self.s))
 
     def arr(self, name: str, count: int | None = None) -> np.ndarray:
         """Copy of an internal array over the live particles."""
         dt, w = _DTYPES.get(name, (np.float64, 1))
         cnt = self.n if count is None else count
-        ptr = lib().or_array(self.s, name.encode())
+        ptr = self.L.or_array(self.s, name.encode())
         if not ptr:
             raise KeyError(name)
         buf = (C.c_char * (cnt * w * np.dtype(dt).itemsize)).from_address(ptr)
@@ -186,7 +188,7 @@ class Oracle:
     def put(self, name: str, values) -> None:
         """Overwrite an internal per-particle array (live particles) in place."""
         dt, w = _DTYPES.get(name, (np.float64, 1))
-        ptr = lib().or_array(self.s, name.encode())
+        ptr = self.L.or_array(self.s, name.encode())
         if not ptr:
             raise KeyError(name)
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=dt),
@@ -195,17 +197,17 @@ class Oracle:
 
     def cstart(self) -> np.ndarray:
         ncell = self.P.cellx * self.P.celly
-        ptr = lib().or_array(self.s, b"cstart")
+        ptr = self.L.or_array(self.s, b"cstart")
         buf = (C.c_int32 * (ncell + 1)).from_address(ptr)
         return np.frombuffer(buf, dtype=np.int32).copy()
 
     def scalars(self) -> Scalars:
         s = Scalars()
-        lib().or_get_scalars(self.s, C.byref(s))
+        self.L.or_get_scalars(self.s, C.byref(s))
         return s
 
     def set_scalars(self, s: Scalars) -> None:
-        lib().or_set_scalars(self.s, C.byref(s))
+        self.L.or_set_scalars(self.s, C.byref(s))
 
     def neighbours(self):
         cnt = self.arr("ncnt")
@@ -213,34 +215,47 @@ class Oracle:
         return [nb[i, :cnt[i]].copy() for i in range(self.n)]
 
     def call(self, name: str, *args):
-        return getattr(lib(), "or_" + name)(self.s, *args)
+        return getattr(self.L, "or_" + name)(self.s, *args)
 
     def ppe_solve(self):
         st = SolveStats()
-        rc = lib().or_ppe_solve(self.s, C.byref(st))
+        rc = self.L.or_ppe_solve(self.s, C.byref(st))
         return rc, st
 
     def set_buffer(self, zone: int, x, y, u, v, p, mass: float, rho: float, vol0: float,
                    c: float, rv: float, interval: float = 0.0, time_shift: float = 0.0) -> int:
         t = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u, v, p)]
-        return lib().or_set_buffer(self.s, zone, t[0].size, *[_ptr(a) for a in t], mass, rho,
+        return self.L.or_set_buffer(self.s, zone, t[0].size, *[_ptr(a) for a in t], mass, rho,
                                    vol0, c, rv, interval, time_shift)
 
     def buffer_update(self, zone: int) -> int:
-        return lib().or_buffer_update(self.s, zone)
+        return self.L.or_buffer_update(self.s, zone)
 
     def set_wavemaker(self, stroke: float, period: float, ramp: float = 0.0,
                       t0: float = 0.0) -> int:
-        return lib().or_set_wavemaker(self.s, stroke, period, ramp, t0)
+        return self.L.or_set_wavemaker(self.s, stroke, period, ramp, t0)
 
     def step(self, nsteps: int = 1):
         st = SolveStats()
-        rc = lib().or_step(self.s, nsteps, C.byref(st))
+        rc = self.L.or_step(self.s, nsteps, C.byref(st))
         return rc, st
 
     def state(self) -> dict:
         out = {k: self.arr(k) for k in PART_FIELDS + ("tid", "flags")}
         return out
+
+
+def build_variant(out_dir, mutant: int) -> pathlib.Path:
+    """Compile the oracle with planted mistake `mutant` (-DOR_MUTANT=k, see the
+    MUT(k) marks in isph_oracle.c) into out_dir; returns the library path.
+    Pin tests use it to show that a check fails on a plausible mistake."""
+    out = pathlib.Path(out_dir) / f"liboracle_m{mutant}.so"
+    cc = os.environ.get("CC", "gcc")
+    cmd = [cc, "-O2", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-std=c11",
+           f"-DOR_MUTANT={int(mutant)}", "-shared", "-o", str(out), str(HERE / "isph_oracle.c"),
+           "-lm"]
+    subprocess.run(cmd, check=True)
+    return out
 
 
 def threads() -> int:

@@ -283,3 +283,482 @@ def test_rebin_preserves_particles_and_orders_cells(oracle, cases):
     assert np.all((x >= (cj - 1) * 4 * h) & (x < cj * 4 * h))
     assert np.all((y >= (celly - 2 - ci) * 4 * h) & (y < (celly - 1 - ci) * 4 * h))
     assert o.n == len(before)
+
+
+# ---- pins of the flow phases, each shown to fail on a planted mistake ------
+# A planted mistake is a build of the oracle with -DOR_MUTANT=k (the MUT(k)
+# marks in oracle/isph_oracle.c): every pin below passes on the oracle and
+# fails on the mutant named next to it.
+MUTANTS = {
+    1: "comp_vel: factor 2 of the fluid pressure term dropped",
+    2: "comp_vel: n_i^2 written as n_i",
+    3: "opt2_shift: shift taken up the concentration gradient",
+    4: "opt2_shift: x and y gradient terms swapped in the velocity correction",
+    5: "effpor: lower-left quadrant walks the right-hand column",
+    6: "effpor: Forchheimer term with n^2 in place of n^3",
+    7: "int_vel: factor 2 of the viscous term dropped",
+    8: "int_vel: sign of the tke term flipped",
+    9: "int_vel: Darcy-Forchheimer resistance explicit instead of implicit",
+    10: "PPE rhs: u*_i - u*_j in place of u*_j - u*_i",
+    11: "PPE rhs: x velocity paired with the y gradient",
+    12: "scalar: arithmetic mean of the diffusivities without the 1/2",
+    13: "scalar: (c_j - c_i) in place of (c_i - c_j)",
+    14: "projection: u times n in place of u / n",
+    15: "comp_pos: projected position in place of the saved one",
+    16: "freesurf: a surface particle flags itself, not its neighbours",
+    17: "int_vel: the y term dropped from the free-surface velocity gradient",
+    18: "PPE: ghost rows take entries for fluid neighbours",
+    19: "int_vel: factor 2 of the SPS harmonic mean dropped",
+}
+
+
+@pytest.fixture(scope="session")
+def mutant(oracle, tmp_path_factory):
+    """mutant(k): the oracle library with planted mistake k."""
+    import concurrent.futures as cf
+    d = tmp_path_factory.mktemp("oracle_mutants")
+    with cf.ThreadPoolExecutor(8) as ex:
+        paths = dict(zip(MUTANTS, ex.map(lambda k: oracle.build_variant(d, k), MUTANTS)))
+    return lambda k: oracle.lib(paths[k])
+
+
+def regular_box(cases, nx=24, ny=20, dx=0.01, jitter=0.0, seed=3, solids=False):
+    """Fluid filling a walled tank [off, off + nx dx] x [off, off + ny dx] (the
+    walls go 4 dx higher), on a regular lattice unless jitter is set.  With
+    `solids`, porous solids on the same lattice cover every interior cell, so
+    each fluid particle's kernel sum exceeds 1 - por and the clamp gives the
+    uniform porosity n = por (porous.f90:182-190)."""
+    P = cases.default_params(dx)
+    P.update(dt=2.0e-4, tol=1e-12, por=0.4, d50=0.01)
+    off = 3.5 * dx
+    wx, wy, wt = cases._walls_box(off, off + nx * dx, off, off + (ny + 4) * dx, dx)
+    gx, gy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    fx = off + (gx.ravel() + 0.5) * dx
+    fy = off + (gy.ravel() + 0.5) * dx
+    if jitter:
+        rng = np.random.default_rng(seed)
+        fx = fx + rng.uniform(-jitter, jitter, fx.size) * dx
+        fy = fy + rng.uniform(-jitter, jitter, fy.size) * dx
+    cases._grid(P, off + (nx + 4) * dx, off + (ny + 8) * dx)
+    sx = sy = None
+    if solids:
+        cs = 4 * P["h"]
+        g1, g2 = np.meshgrid(np.arange(int((P["cellx"] - 2) * cs / dx)),
+                             np.arange(int((P["celly"] - 2) * cs / dx)), indexing="ij")
+        sx = (g1.ravel() + 0.5) * dx
+        sy = (g2.ravel() + 0.5) * dx
+    return cases._assemble(P, "regular_box", fx, fy, np.zeros(fx.size), wx, wy, wt, dx,
+                           off + ny * dx, sx, sy, meta=dict(nx=nx, ny=ny, dx=dx, off=off))
+
+
+def interior(c, o, away_from_walls):
+    """Fluid particles whose 2h support lies below the free surface (and, with
+    away_from_walls, holds no wall particle)."""
+    x, y, tid = o.arr("x"), o.arr("y"), o.arr("tid")
+    m, h = c.meta, c.params["h"]
+    top = m["off"] + m["ny"] * m["dx"]
+    k = (tid == 3) & (y < top - 2 * h - m["dx"])
+    if away_from_walls:
+        k &= (x > m["off"] + 2 * h + m["dx"]) & (x < m["off"] + m["nx"] * m["dx"] - 2 * h - m["dx"])
+        k &= y > m["off"] + 2 * h + m["dx"]
+    return k
+
+
+def porous_state(oracle, c, L):
+    """init_state, then neighbour and effpor again, so the correction matrix
+    is built with the porous particle volumes m/rho = V0/n (the order of a
+    production step: effpor's density feeds the next neighbour search)."""
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    if c.ns:
+        assert o.call("neighbour") == 0
+        o.call("effpor")
+    return o
+
+
+# comp_vel, integrator.f90:705-843 ------------------------------------------
+def hydrostatic_residual(oracle, cases, L, solids):
+    """Still water: int_vel gives v* = g n dt; with the hydrostatic pressure
+    p = rho0 |g| (H - y) comp_vel must return u = v = 0, since the pressure
+    term is (n/rho0) grad p for a linear p and uniform n."""
+    c = regular_box(cases, solids=solids)
+    o = porous_state(oracle, c, L)
+    y = o.arr("y")
+    top = c.meta["off"] + c.meta["ny"] * c.meta["dx"]
+    o.put("p", c.params["rho"] * abs(c.params["g"]) * (top - y))
+    o.put("nut", 0.0)
+    o.put("tke", 0.0)
+    o.call("int_vel")
+    o.call("comp_vel")
+    k = interior(c, o, away_from_walls=solids)
+    assert k.sum() > 100
+    if solids:
+        assert np.allclose(o.arr("por")[k], o.arr("por")[k][0], rtol=1e-12)
+        assert o.arr("por")[k][0] < 0.5
+    return max(np.abs(o.arr("u")[k]).max(), np.abs(o.arr("v")[k]).max())
+
+
+def test_comp_vel_keeps_hydrostatic_water_at_rest(oracle, cases, mutant):
+    # n = 1 (walls included in the support) and uniform n < 1 (away from the
+    # walls, whose volume m/rho stays V0): |u| <= 1e-12 against g dt ~ 2e-3
+    L = oracle.lib()
+    assert hydrostatic_residual(oracle, cases, L, False) < 1e-12
+    assert hydrostatic_residual(oracle, cases, L, True) < 1e-12
+    assert hydrostatic_residual(oracle, cases, mutant(1), False) > 1e-4
+    assert hydrostatic_residual(oracle, cases, mutant(2), True) > 1e-4
+
+
+# projection isph.f90:30-37 and comp_pos integrator.f90:859-863 --------------
+def kinematics_error(oracle, cases, L, solids):
+    """projection with u_old, then comp_pos with u_new = u_old + a dt: the
+    trapezoid rule integrates a constant acceleration exactly, and particles
+    move at the pore velocity u/n: x = x_old + (u_old dt + a dt^2 / 2) / n."""
+    c = regular_box(cases, nx=16, ny=12, jitter=0.05, solids=solids)
+    o = porous_state(oracle, c, L)
+    fl = o.arr("tid") == 3
+    x0, y0, n = o.arr("x"), o.arr("y"), o.arr("por")
+    U, V, ax, ay, dt = 0.2, -0.1, 3.0, -5.0, c.params["dt"]
+    o.put("u", U)
+    o.put("v", V)
+    o.call("projection")
+    o.put("u", U + ax * dt)
+    o.put("v", V + ay * dt)
+    o.call("comp_pos")
+    x1, y1 = o.arr("x"), o.arr("y")
+    assert np.all(x1[~fl] == x0[~fl]) and np.all(y1[~fl] == y0[~fl])
+    ex = np.abs(x1 - (x0 + (U * dt + 0.5 * ax * dt * dt) / n))[fl]
+    ey = np.abs(y1 - (y0 + (V * dt + 0.5 * ay * dt * dt) / n))[fl]
+    return max(ex.max(), ey.max())
+
+
+def test_projection_and_comp_pos_integrate_constant_acceleration(oracle, cases, mutant):
+    L = oracle.lib()
+    for solids in (False, True):
+        assert kinematics_error(oracle, cases, L, solids) < 1e-15
+    assert kinematics_error(oracle, cases, mutant(14), True) > 1e-6
+    assert kinematics_error(oracle, cases, mutant(15), False) > 1e-6
+
+
+# freesurf, isph.f90:49-86 ------------------------------------------------------
+def vicinity_mismatches(oracle, cases, L):
+    """The vicinity flags equal a brute-force search: every particle within
+    2h of a non-buffer fluid particle with gradvx <= fs_thresh."""
+    c = cases.tiny_box(nx=14, ny=12, solids=False, patch=False)
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    before = o.arr("flags").copy()
+    o.call("freesurf")
+    x, y, tid, gv = o.arr("x"), o.arr("y"), o.arr("tid"), o.arr("gradvx")
+    surf = np.flatnonzero((tid == 3) & ((before & 1) == 0) & (gv <= c.params["fs_thresh"]))
+    assert surf.size > 10
+    want = (before & 4) > 0
+    for i in surf:
+        near = np.hypot(x[i] - x, y[i] - y) < 2 * c.params["h"]
+        near[i] = False
+        want |= near
+    return int(np.sum(((o.arr("flags") & 4) > 0) != want))
+
+
+def test_freesurf_vicinity_matches_brute_force(oracle, cases, mutant):
+    assert vicinity_mismatches(oracle, cases, oracle.lib()) == 0
+    assert vicinity_mismatches(oracle, cases, mutant(16)) > 0
+
+
+# opt2_shift, part_shift.f90:12-191 ---------------------------------------------
+def shift_errors(oracle, cases, L):
+    """The shift is -csh h^2 frac grad C, C(x) = sum_j V_j W(|x - x_j|) the
+    particle concentration (grad C by central differences of the kernel sum,
+    an independent route to the analytic gradient sum), capped at
+    maxshift dl; and for a linear velocity field the velocity correction is
+    exactly grad u . shift.  Returns the worst shift error relative to the
+    cap and the worst correction error relative to |shift|."""
+    c = regular_box(cases, nx=16, ny=12, jitter=0.08)
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    x, y, tid, flags = o.arr("x"), o.arr("y"), o.arr("tid"), o.arr("flags")
+    V = o.arr("m") / o.arr("rho")
+    a, b, cc, d = 0.3, -0.2, 0.15, 0.25
+    o.put("u", a * x + b * y)
+    o.put("v", cc * x + d * y)
+    o.call("opt2_shift")
+    xs, ys, us, vs = o.arr("xs"), o.arr("ys"), o.arr("us"), o.arr("vs")
+    P = c.params
+    h, cap, e = P["h"], P["maxshift"] * P["dl"], 1e-6 * P["h"]
+    ws = wl = 0.0
+    for i in np.flatnonzero(tid == 3):
+        js = np.flatnonzero((np.hypot(x[i] - x, y[i] - y) < 2 * h) & (np.arange(x.size) != i))
+
+        def conc(px, py):
+            return math.fsum(V[j] * L.or_wab(math.hypot(px - x[j], py - y[j]), h) for j in js)
+        gx = (conc(x[i] + e, y[i]) - conc(x[i] - e, y[i])) / (2 * e)
+        gy = (conc(x[i], y[i] + e) - conc(x[i], y[i] - e)) / (2 * e)
+        frac = 0.01 if flags[i] & 2 else 1.0
+        dx, dy = -P["csh"] * h * h * frac * gx, -P["csh"] * h * h * frac * gy
+        ws = max(ws, abs(xs[i] - math.copysign(min(abs(dx), cap), dx)) / cap,
+                 abs(ys[i] - math.copysign(min(abs(dy), cap), dy)) / cap)
+        if abs(dx) < cap and abs(dy) < cap:
+            sc = abs(xs[i]) + abs(ys[i])
+            wl = max(wl, abs(us[i] - (a * xs[i] + b * ys[i])) / sc,
+                     abs(vs[i] - (cc * xs[i] + d * ys[i])) / sc)
+    return ws, wl
+
+
+def test_shift_follows_concentration_gradient_and_corrects_velocity(oracle, cases, mutant):
+    ws, wl = shift_errors(oracle, cases, oracle.lib())
+    assert ws < 1e-7 and wl < 1e-12
+    assert shift_errors(oracle, cases, mutant(3))[0] > 0.1
+    assert shift_errors(oracle, cases, mutant(4))[1] > 0.1
+
+
+# effpor / porsearch, porous.f90:32-224 -----------------------------------------
+def porosity_errors(oracle, cases, L, lamk):
+    """Porosity against a brute-force sum over every solid within 2h (no
+    cell walk): n = 1 - min(sum (1 - por) W m_s / rho_s, 1 - por); and with
+    lamk = 150 the resistance is Ergun's equation per unit mass (Ergun 1952):
+    -(150 nu (1-n)^2 / (n^3 d^2) + 1.75 (1-n) |u| / (n^3 d)), nu = mu(rho0)/rho0.
+    Returns the worst porosity difference, the worst relative resistance
+    difference, and the counts of porous and clamped particles."""
+    c = cases.tiny_box(nx=14, ny=12, solids=True, patch=False)
+    c.params["lamk"] = lamk
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    x, y = o.arr("x"), o.arr("y")
+    fl = np.flatnonzero(o.arr("tid") == 3)
+    S = c.solids
+    h, por, d = c.params["h"], c.params["por"], c.params["d50"]
+    nb = np.empty(fl.size)
+    for q, i in enumerate(fl):
+        r = np.hypot(x[i] - S["x"], y[i] - S["y"])
+        sf = math.fsum((1.0 - por) * L.or_wab(r[j], h) * S["m"][j] / S["rho"][j]
+                       for j in np.flatnonzero(r < 2 * h))
+        nb[q] = 1.0 - min(sf, 1.0 - por)
+    n = o.arr("por")[fl]
+    u, v, rho0 = o.arr("u")[fl], o.arr("v")[fl], o.arr("rho0")[fl]
+    nu = np.array([L.or_vis(r0) for r0 in rho0]) / rho0
+    ergun = -(150.0 * nu * (1 - n) ** 2 / (n ** 3 * d * d) +
+              1.75 * (1 - n) * np.hypot(u, v) / (n ** 3 * d))
+    pm = n < 1.0
+    er = np.abs(o.arr("resist")[fl][pm] / ergun[pm] - 1).max()
+    return np.abs(n - nb).max(), er, int(pm.sum()), int(np.sum(n == por))
+
+
+def test_effpor_matches_brute_force_and_ergun(oracle, cases, mutant):
+    en, er, npor, nclamp = porosity_errors(oracle, cases, oracle.lib(), 150.0)
+    assert npor > 20 and nclamp > 5 and en < 1e-14 and er < 1e-12
+    assert porosity_errors(oracle, cases, mutant(5), 150.0)[0] > 1e-3
+    assert porosity_errors(oracle, cases, mutant(6), 150.0)[1] > 1e-2
+
+
+# int_vel, integrator.f90:459-703 ------------------------------------------------
+def viscous_ratio(oracle, cases, L, nut):
+    """u = 3 x^2, v = -2 y^2 (Laplacians 6 and -4), uniform nu_t: away from
+    walls and surface, (u* - u) / (dt (nu + nu_t)) is the Laplacian, to the
+    operator's lattice error (the 1e-3 h^2 regulariser costs ~0.1%).
+    Returns the range of the ratio to the exact Laplacian."""
+    c = regular_box(cases, nx=30, ny=24)
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    x, y = o.arr("x"), o.arr("y")
+    u0, v0 = 3.0 * x * x, -2.0 * y * y
+    o.put("u", u0)
+    o.put("v", v0)
+    o.put("nut", nut)
+    o.put("tke", 0.0)
+    o.call("int_vel")
+    k = interior(c, o, away_from_walls=True)
+    nu = L.or_vis(1000.0) / 1000.0
+    dt, g = c.params["dt"], c.params["g"]
+    r = np.concatenate([(o.arr("us")[k] - u0[k]) / (dt * (nu + nut)) / 6.0,
+                        (o.arr("vs")[k] - v0[k] - g * dt) / (dt * (nu + nut)) / -4.0])
+    return r.min(), r.max()
+
+
+def test_int_vel_viscous_and_sps_terms_are_the_laplacian(oracle, cases, mutant):
+    # nu_t below the real4 1e-6 threshold (arithmetic mean) and above it
+    # (harmonic mean) both reduce to nu_t for a uniform field
+    for nut in (0.0, 5e-7, 2e-5):
+        lo, hi = viscous_ratio(oracle, cases, oracle.lib(), nut)
+        assert 0.99 < lo <= hi < 1.01, (nut, lo, hi)
+    assert viscous_ratio(oracle, cases, mutant(7), 0.0)[0] < 0.9
+    assert viscous_ratio(oracle, cases, mutant(19), 2e-5)[0] < 0.9
+
+
+def tke_error(oracle, cases, L):
+    """u = v = 0, k = 0.7 x - 0.4 y: the tke term -(2/3) n grad k is exact
+    for a linear k (corrected gradient), so u* = -(2/3)(0.7) dt and
+    v* = -(2/3)(-0.4) dt + g dt for every fluid particle."""
+    c = regular_box(cases, jitter=0.05)
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    x, y = o.arr("x"), o.arr("y")
+    o.put("u", 0.0)
+    o.put("v", 0.0)
+    o.put("nut", 0.0)
+    o.put("tke", 0.7 * x - 0.4 * y)
+    o.call("int_vel")
+    fl = o.arr("tid") == 3
+    dt, g = c.params["dt"], c.params["g"]
+    eu = o.arr("us")[fl] / (-(2.0 / 3.0) * 0.7 * dt) - 1
+    ev = (o.arr("vs")[fl] - g * dt) / (-(2.0 / 3.0) * -0.4 * dt) - 1
+    return max(np.abs(eu).max(), np.abs(ev).max())
+
+
+def test_int_vel_tke_term_is_the_gradient(oracle, cases, mutant):
+    assert tke_error(oracle, cases, oracle.lib()) < 1e-9
+    assert tke_error(oracle, cases, mutant(8)) > 0.5
+
+
+def darcy_error(oracle, cases, L):
+    """Uniform flow U through uniform porosity: every flow term vanishes and
+    int_vel is the implicit Euler step of du/dt = R n u, u* = U / (1 - dt R n)."""
+    c = regular_box(cases, solids=True)
+    o = porous_state(oracle, c, L)
+    U = 0.3
+    o.put("u", U)
+    o.put("v", 0.0)
+    o.put("nut", 0.0)
+    o.put("tke", 0.0)
+    o.call("effpor")  # resistance at the new speed
+    o.call("int_vel")
+    fl = o.arr("tid") == 3
+    R, n, dt = o.arr("resist")[fl], o.arr("por")[fl], c.params["dt"]
+    assert R.max() < -100.0
+    return np.abs(o.arr("us")[fl] * (1.0 - dt * R * n) / U - 1).max()
+
+
+def test_int_vel_darcy_forchheimer_is_implicit(oracle, cases, mutant):
+    assert darcy_error(oracle, cases, oracle.lib()) < 1e-14
+    assert darcy_error(oracle, cases, mutant(9)) > 1e-4
+
+
+def varts_error(oracle, cases, L):
+    """Free-surface particles with u = 0.4 x, v = -0.3 y (n = 1): the
+    corrected velocity gradient is exact, so
+    varts = dl^2 |(0.4, -0.3)| |u_i| / ((umax + 1e-6) n)."""
+    c = regular_box(cases, jitter=0.05)
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    x, y = o.arr("x"), o.arr("y")
+    o.put("u", 0.4 * x)
+    o.put("v", -0.3 * y)
+    o.put("nut", 0.0)
+    o.put("tke", 0.0)
+    umax = o.scalars().umax
+    o.call("int_vel")
+    free = (o.arr("flags") & 2) > 0
+    assert free.sum() > 10
+    dl = c.params["dl"]
+    want = dl * dl * 0.5 * np.hypot(0.4 * x, -0.3 * y) / (
+        (umax + float(np.float32(1e-6))) * o.arr("por"))
+    return np.abs(o.arr("varts")[free] / want[free] - 1).max()
+
+
+def test_int_vel_varts_is_the_velocity_gradient(oracle, cases, mutant):
+    assert varts_error(oracle, cases, oracle.lib()) < 1e-12
+    assert varts_error(oracle, cases, mutant(17)) > 0.1
+
+
+# PPE right-hand side and wall rows, isph.f90:208-410 ---------------------------------
+def _row_apply(ln, col, val, q):
+    return np.array([np.dot(val[i, :ln[i]], q[col[i, :ln[i]]]) for i in range(len(ln))])
+
+
+def ppe_rhs_checks(oracle, cases, L, jitter):
+    """u* linear, n = 1: the raw source sum_j V_j (u*_j - u*_i) rho / dt . grad W~
+    is rho div(u*) / dt exactly, so after the Jacobi scaling every non-free
+    fluid row's rhs is proportional to div(u*) (ratio of two fields exact,
+    zero for a divergence-free field).  The scale: the same row applied to
+    q = x^2 + y^2 gives lap(q) / diag = 4 / diag, so rhs / (A q) =
+    rho div / (4 dt) to the Laplacian's lattice error (regular lattice).
+    Returns (ratio error, divergence-free residual, scale range)."""
+    c = regular_box(cases, nx=30, ny=24, jitter=jitter)
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    x, y, tid = o.arr("x"), o.arr("y"), o.arr("tid")
+    dt, rho = c.params["dt"], c.params["rho"]
+    rhs = {}
+    for name, (a, b, cc, d) in {"f1": (0.5, 0.2, -0.3, 0.9), "f2": (0.25, 0.7, 0.1, 0.0),
+                                "free": (0.6, 0.2, -0.3, -0.6)}.items():
+        o.put("us", a * x + b * y)
+        o.put("vs", cc * x + d * y)
+        assert o.call("ppe_assemble") == 0
+        rhs[name] = o.arr("b").copy()
+    nf = (tid == 3) & ((o.arr("flags") & 2) == 0)
+    ratio = np.abs(rhs["f1"][nf] * 0.25 / (rhs["f2"][nf] * 1.4) - 1).max()
+    zero = np.abs(rhs["free"][nf]).max() / np.abs(rhs["f1"][nf]).max()
+    Aq = _row_apply(o.arr("alen"), o.arr("acol"), o.arr("aval"), x * x + y * y)
+    k = interior(c, o, away_from_walls=False) & nf
+    sc = rhs["f1"][k] / Aq[k] / (rho * 1.4 / (4 * dt))
+    return ratio, zero, (sc.min(), sc.max())
+
+
+def test_ppe_rhs_is_the_scaled_divergence(oracle, cases, mutant):
+    ratio, zero, _ = ppe_rhs_checks(oracle, cases, oracle.lib(), 0.05)
+    assert ratio < 1e-12 and zero < 1e-12
+    _, _, (lo, hi) = ppe_rhs_checks(oracle, cases, oracle.lib(), 0.0)
+    assert 0.99 < lo <= hi < 1.01
+    lo, hi = ppe_rhs_checks(oracle, cases, mutant(10), 0.0)[2]
+    assert not (0.99 < lo <= hi < 1.01)
+    r11 = ppe_rhs_checks(oracle, cases, mutant(11), 0.05)
+    assert r11[0] > 0.1 and r11[1] > 0.1
+
+
+def wall_row_checks(oracle, cases, L):
+    """Ghost rows (isph.f90:272-293) couple only to wall particles, sum to
+    zero and have no source; fixed-wall rows (isph.f90:302-410) sum to zero
+    (the diagonal is the sum of the off-diagonal terms in both)."""
+    c = regular_box(cases, jitter=0.05)
+    o = oracle.Oracle(c, library=L)
+    assert o.call("init_state") == 0
+    x, y = o.arr("x"), o.arr("y")
+    o.put("us", 0.3 * x - 0.1 * y)
+    o.put("vs", 0.2 * y)
+    assert o.call("ppe_assemble") == 0
+    tid, ln, col, val, b = (o.arr(k) for k in ("tid", "alen", "acol", "aval", "b"))
+    gh, fx = np.flatnonzero(tid == 2), np.flatnonzero(tid == 1)
+    only_walls = all(np.all(tid[col[i, :ln[i]]] <= 2) for i in gh)
+    sums = max(abs(val[i, :ln[i]].sum()) for i in np.concatenate([gh, fx]))
+    return only_walls, sums, np.abs(b[gh]).max()
+
+
+def test_ppe_wall_rows(oracle, cases, mutant):
+    only_walls, sums, ghost_rhs = wall_row_checks(oracle, cases, oracle.lib())
+    assert only_walls and sums < 1e-13 and ghost_rhs == 0.0
+    assert not wall_row_checks(oracle, cases, mutant(18))[0]
+
+
+# scalart_anisotropic, scalar.f90:242-362 ---------------------------------------
+def scalar_ratio(oracle, cases, L, solids, quadratic=True):
+    """u = v = 0.  Without solids (n = 1 >= 0.7) the coefficient is the
+    molecular Dm (arithmetic mean, nu_t = 0); with solids everywhere
+    (n = por < 0.7) the anisotropic tensor is 0/0 at zero velocity and the
+    cap 0.12 dl^2 / dt applies (merge, scalar.f90:311).  For c = 3(x^2 + y^2)
+    cdiff n / (D lap c) is 1 to the lattice error of the uncorrected kernel
+    Laplacian (returns its range); for a linear c cdiff is 0 to round-off
+    (returns max |cdiff| / (D |grad c| / h))."""
+    c = regular_box(cases, nx=30, ny=24, solids=solids)
+    o = porous_state(oracle, c, L)
+    x, y, tid = o.arr("x"), o.arr("y"), o.arr("tid")
+    o.put("u", 0.0)
+    o.put("v", 0.0)
+    o.put("nut", 0.0)
+    o.put("c", 3.0 * (x * x + y * y) if quadratic else 0.7 * x - 0.4 * y)
+    o.call("scalart")
+    k = interior(c, o, away_from_walls=True) & (tid == 3)
+    P = c.params
+    D = 0.12 * P["dl"] ** 2 / P["dt"] if solids else float(np.float32(1e-9))
+    cd, n = o.arr("cdiff")[k], o.arr("por")[k]
+    if quadratic:
+        r = cd * n / (D * 12.0)
+        return r.min(), r.max()
+    return np.abs(cd).max() / (D * 0.81 / P["h"])
+
+
+def test_scalar_transport_is_the_laplacian_in_both_branches(oracle, cases, mutant):
+    L = oracle.lib()
+    for solids in (False, True):
+        lo, hi = scalar_ratio(oracle, cases, L, solids)
+        assert 0.95 < lo <= hi < 1.02, (solids, lo, hi)
+        assert scalar_ratio(oracle, cases, L, solids, quadratic=False) < 1e-12
+    assert scalar_ratio(oracle, cases, mutant(12), False)[1] > 1.5
+    assert scalar_ratio(oracle, cases, mutant(13), True)[1] < 0
