@@ -25,6 +25,7 @@ LIB_PATH = HERE / "libisph_hip.so"
 
 NBR_CAP = 64
 ROW_CAP = NBR_CAP + 2
+NOALLOC_MAX_ROWS = 1 << 20   # ISPH_NOALLOC_MAX_ROWS
 PPE_FP64, PPE_FP32_REFINE = 0, 1
 
 ERRORS = {-1: "HIP error", -2: "bad argument", -3: "capacity overflow",
@@ -85,7 +86,8 @@ EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_load_system", "isph_batch_create", "isph_batch_step", "isph_batch_stats",
            "isph_batch_destroy", "isph_set_buffer", "isph_buffer_update", "isph_set_wcsph",
            "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read",
-           "isph_wavemaker_motion", "isph_set_wavemaker", "isph_wavemaker") + tuple("isph_" + p for p in PHASES)
+           "isph_wavemaker_motion", "isph_set_wavemaker", "isph_wavemaker",
+           "isph_ppe_solve_noalloc") + tuple("isph_" + p for p in PHASES)
 
 _lib = None
 
@@ -141,6 +143,8 @@ def lib():
     L.isph_prof_read.argtypes = [vp, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
     L.isph_footprint_bytes.restype = i64
     L.isph_footprint_bytes.argtypes = [i64, i64, i32, i32, i32]
+    L.isph_ppe_solve_noalloc.argtypes = [i32, i32] + [vp] * 5 + [d, i32, i32, vp,
+                                                                C.POINTER(SolveStats)]
     _lib = L
     return L
 
@@ -431,3 +435,20 @@ def wavemaker_motion(stroke: float, period: float, ramp: float, t0: float, t: fl
 def footprint_bytes(n_cap: int, n_solid_cap: int, cellx: int, celly: int,
                     precision: int = 0) -> int:
     return int(lib().isph_footprint_bytes(n_cap, n_solid_cap, cellx, celly, precision))
+
+
+def ppe_solve_noalloc(n: int, ld: int, row_len: int, col: int, val: int, rhs: int, x: int,
+                      tol: float, max_iter: int, max_outer: int = 0, stream: int | None = None,
+                      check: bool = True):
+    """FP64 BiCGStab on a system already in device memory, with no allocation
+    (isph_ppe_solve_noalloc).  row_len, col, val, rhs and x are device
+    addresses (integers); col/val are ELL slot-major with leading dimension
+    ld; x holds the initial guess and receives the solution.  Returns
+    (return code, SolveStats)."""
+    st = SolveStats()
+    rc = lib().isph_ppe_solve_noalloc(int(n), int(ld), row_len, col, val, rhs, x, float(tol),
+                                      int(max_iter), int(max_outer), stream, C.byref(st))
+    if check and rc != 0:
+        raise IsphError(rc, "isph_ppe_solve_noalloc",
+                        (lib().isph_last_error(None) or b"").decode())
+    return rc, st

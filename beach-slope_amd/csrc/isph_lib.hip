@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -91,9 +92,10 @@ static void set_err(isph_ctx* c, const char* fmt, ...) {
   c->err = buf;
 }
 
-// message of the calling thread's last failed isph_create (no context exists
-// to hold it); returned by isph_last_error(NULL)
-static thread_local std::string g_create_err;
+// message of the calling thread's last failed isph_create or
+// isph_ppe_solve_noalloc (no context exists to hold it); returned by
+// isph_last_error(NULL)
+static thread_local std::string g_noctx_err;
 
 static int create_fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -101,7 +103,7 @@ static int create_fail(int code, const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  g_create_err = buf;
+  g_noctx_err = buf;
   return code;
 }
 
@@ -179,7 +181,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
                            isph_ctx** out) {
   // slot-major ELL indices k*cap + i are int32: cap*ROW_CAP must stay below 2^31
   if (out) *out = nullptr;
-  g_create_err.clear();
+  g_noctx_err.clear();
   if (!params || !out) return create_fail(ISPH_EARG, "isph_create: null argument");
   if (n_cap <= 0 || n_cap * ROW_CAP >= (1ll << 31) || n_solid_cap < 0)
     return create_fail(ISPH_EARG, "isph_create: capacity %lld outside (0, 2^31/%d)",
@@ -338,7 +340,7 @@ extern "C" int isph_destroy(isph_ctx* c) {
 
 extern "C" const char* isph_last_error(const isph_ctx* c) {
   if (c) return c->err.c_str();
-  return g_create_err.empty() ? "null context" : g_create_err.c_str();
+  return g_noctx_err.empty() ? "null context" : g_noctx_err.c_str();
 }
 
 extern "C" int isph_set_params(isph_ctx* c, const isph_params* p) {
@@ -1835,5 +1837,236 @@ extern "C" int isph_batch_destroy(isph_batch* b) {
   bh_free(b->pb);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// allocation-free FP64 solve (isph_ppe_solve_noalloc)
+// ---------------------------------------------------------------------
+// Scratch of the allocation-free solve, one static device array per GPU:
+// six BiCGStab vectors of ISPH_NOALLOC_MAX_ROWS doubles (r, rh, p, t1, s,
+// t2), the block partials of two dot products, then the solver scalars and
+// the squared right-hand-side norm.  Host code never uses the array's name
+// as an address: scratch_base() looks up each device's own copy once, with
+// that device current, and every kernel argument is derived from it.
+#define NA_ROWS ((size_t)ISPH_NOALLOC_MAX_ROWS)
+#define NA_PART (2 * (NA_ROWS / RED_BLOCK + 2))
+#define NA_SC_DOUBLES 48
+static_assert(sizeof(DevScalars) + sizeof(double) <= NA_SC_DOUBLES * sizeof(double),
+              "solver scalars do not fit their scratch slot");
+static __device__ __align__(256) double scratch[6 * NA_ROWS + NA_PART + NA_SC_DOUBLES];
+
+#define NA_MAX_DEVICES 64
+static std::mutex g_na_mu;
+static double* g_na_base[NA_MAX_DEVICES];
+
+static int na_fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_noctx_err = buf;
+  return code;
+}
+
+// this device's copy of the scratch array (looked up on its first use)
+static int scratch_base(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return na_fail(ISPH_EHIP, "isph_ppe_solve_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= NA_MAX_DEVICES)
+    return na_fail(ISPH_EHIP, "isph_ppe_solve_noalloc: device %d beyond %d", dev, NA_MAX_DEVICES);
+  std::lock_guard<std::mutex> g(g_na_mu);
+  if (!g_na_base[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return na_fail(ISPH_EHIP, "isph_ppe_solve_noalloc: hipGetSymbolAddress: %s",
+                     hipGetErrorString(e));
+    g_na_base[dev] = static_cast<double*>(p);
+  }
+  *out = g_na_base[dev];
+  return 0;
+}
+
+// the solver scalars start from zero with the row count set
+__global__ void k_na_begin(DevScalars* sc, int n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    DevScalars z{};
+    z.n = n;
+    *sc = z;
+  }
+}
+
+// every row length within the capacity and every column inside [0, n)
+__global__ void __launch_bounds__(256) k_na_check(Dev D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = D.sc->n;
+  if (i >= n) return;
+  const int L = D.alen[i];
+  bool bad = L < 0 || L > ROW_CAP;
+  for (int k = 0; !bad && k < L; k++) {
+    const int j = D.acol[(size_t)k * D.cap + i];
+    bad = j < 0 || j >= n;
+  }
+  if (bad) atomicOr(&D.sc->err, 1);
+}
+
+__global__ void k_na_set_bnorm(DevScalars* sc, double bnorm) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->bnorm = bnorm;
+}
+
+__global__ void k_na_zero(double* x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+#define NA_LAUNCH(kern, grid, ...)                                                   \
+  do {                                                                               \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, __VA_ARGS__);             \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess)                                                            \
+      return na_fail(ISPH_EHIP, "isph_ppe_solve_noalloc: launch of %s failed: %s",   \
+                     #kern, hipGetErrorString(e_));                                  \
+  } while (0)
+
+static int na_read(hipStream_t st, const DevScalars* d_sc, DevScalars* s, double* bn2) {
+  hipError_t e = hipMemcpyAsync(s, d_sc, sizeof *s, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && bn2)
+    e = hipMemcpyAsync(bn2, reinterpret_cast<const double*>(d_sc) + NA_SC_DOUBLES - 1,
+                       sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess)
+    return na_fail(ISPH_EHIP, "isph_ppe_solve_noalloc: read-back: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" int isph_ppe_solve_noalloc(int32_t n, int32_t ld, const int32_t* row_len,
+                                      const int32_t* col, const double* val, const double* rhs,
+                                      double* x, double tol, int32_t max_iter,
+                                      int32_t max_outer, void* stream,
+                                      isph_solve_stats* stats) {
+  g_noctx_err.clear();
+  if (n < 0 || (size_t)n > NA_ROWS)
+    return na_fail(ISPH_EARG, "isph_ppe_solve_noalloc: %d rows outside [0, %d]", n,
+                   ISPH_NOALLOC_MAX_ROWS);
+  if (ld < n || (int64_t)ld * ROW_CAP >= (1ll << 31))
+    return na_fail(ISPH_EARG, "isph_ppe_solve_noalloc: leading dimension %d below %d rows "
+                   "or beyond 2^31/%d", ld, n, ROW_CAP);
+  if (!row_len || !col || !val || !rhs || !x)
+    return na_fail(ISPH_EARG, "isph_ppe_solve_noalloc: null device pointer");
+  if (!(tol > 0.0) || max_iter < 0 || max_outer < 0)
+    return na_fail(ISPH_EARG, "isph_ppe_solve_noalloc: tol %g, max_iter %d, max_outer %d",
+                   tol, max_iter, max_outer);
+  isph_solve_stats out{};
+  if (n == 0) {
+    out.converged = 1;
+    if (stats) *stats = out;
+    return 0;
+  }
+  double* base = nullptr;
+  int rc = scratch_base(&base);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Dev D{};
+  D.P.tol = tol;
+  D.P.max_iter = max_iter;
+  D.P.max_outer = max_outer;
+  D.cap = ld;
+  D.nblk = grid_of(n);
+  D.alen = const_cast<int32_t*>(row_len);
+  D.acol = const_cast<int32_t*>(col);
+  D.aval = const_cast<double*>(val);
+  D.b = const_cast<double*>(rhs);
+  D.sol = x;
+  D.r = base;
+  D.rh = base + NA_ROWS;
+  D.pp = base + 2 * NA_ROWS;
+  D.t1 = base + 3 * NA_ROWS;
+  D.sv = base + 4 * NA_ROWS;
+  D.t2 = base + 5 * NA_ROWS;
+  D.part = base + 6 * NA_ROWS;
+  D.sc = reinterpret_cast<DevScalars*>(base + 6 * NA_ROWS + NA_PART);
+  double* d_bn2 = base + 6 * NA_ROWS + NA_PART + NA_SC_DOUBLES - 1;
+  const int rg = row_grid(n), nb = grid_of(n);
+  // clear the scalars the previous call left; check the rows; ||b||^2
+  NA_LAUNCH(k_na_begin, 1, D.sc, n);
+  NA_LAUNCH(k_na_check, grid_of(n), D);
+  NA_LAUNCH(k_dot, rg, D, D.b, D.b);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, D, (int)FIN_DOT, nb, d_bn2);
+  DevScalars s;
+  double bn2 = 0.0;
+  rc = na_read(st, D.sc, &s, &bn2);
+  if (rc) return rc;
+  if (s.err)
+    return na_fail(ISPH_EARG, "isph_ppe_solve_noalloc: a row is longer than %d entries or "
+                   "has a column outside [0, %d)", ROW_CAP, n);
+  const double bnorm = std::sqrt(bn2);
+  out.bnorm = bnorm;
+  if (bnorm == 0.0 || !std::isfinite(bnorm)) {
+    if (bnorm == 0.0) {
+      NA_LAUNCH(k_na_zero, grid_of(n), x, n);
+      out.converged = 1;
+    }
+    out.rel_residual = (bnorm == 0.0) ? 0.0 : bnorm;
+  } else {
+    // the FP64 iteration of isph_ppe_solve: same launches, same chunked
+    // polling, same restarts from the true residual after a breakdown
+    NA_LAUNCH(k_na_set_bnorm, 1, D.sc, bnorm);
+    NA_LAUNCH(k_bicg_init, rg, D);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, D, (int)FIN_INIT, nb,
+                       (double*)nullptr);
+    rc = na_read(st, D.sc, &s, nullptr);
+    if (rc) return rc;
+    int restarts = 0, chunk = 4;
+    for (int64_t it = 0; !s.done;) {
+      for (int k = 0; k < chunk; k++) {
+        NA_LAUNCH(k_bicg_1, rg, D);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, D, (int)FIN_ALPHA, nb,
+                           (double*)nullptr);
+        NA_LAUNCH(k_bicg_2, rg, D);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, D, (int)FIN_OMEGA, nb,
+                           (double*)nullptr);
+        NA_LAUNCH(k_bicg_3, rg, D);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, D, (int)FIN_BETA, nb,
+                           (double*)nullptr);
+        NA_LAUNCH(k_bicg_4, rg, D);
+      }
+      it += chunk;
+      rc = na_read(st, D.sc, &s, nullptr);
+      if (rc) return rc;
+      if (s.done) {
+        if (s.brk && !s.conv && s.iters < max_iter && restarts < max_outer) {
+          restarts++;
+          NA_LAUNCH(k_bicg_init, rg, D);
+          hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, D, (int)FIN_RESTART, nb,
+                             (double*)nullptr);
+          rc = na_read(st, D.sc, &s, nullptr);
+          if (rc) return rc;
+          chunk = 4;
+          continue;
+        }
+        break;
+      }
+      if (chunk < 32) chunk *= 2;
+      if (it > (int64_t)(max_iter + 64) * (max_outer + 1)) break;
+    }
+    out.iterations = s.iters;
+    out.outer = restarts;
+    out.breakdown = s.brk;
+    out.converged = s.conv;
+    out.rel_residual = s.rel;
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess)
+    return na_fail(ISPH_EHIP, "isph_ppe_solve_noalloc: %s", hipGetErrorString(e));
+  if (stats) *stats = out;
+  if (!std::isfinite(out.rel_residual) || !std::isfinite(out.bnorm))
+    return na_fail(ISPH_ENONFINITE, "isph_ppe_solve_noalloc: non-finite residual");
+  if (!out.converged)
+    return na_fail(ISPH_ENOCONV, "isph_ppe_solve_noalloc: %d iterations, rel residual %g",
+                   out.iterations, out.rel_residual);
   return 0;
 }

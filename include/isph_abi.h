@@ -309,6 +309,36 @@ int isph_prof_read(isph_ctx *ctx, const char *kernel, double *total_ms,
 int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap,
                              int32_t cellx, int32_t celly, int32_t precision);
 
+/* ---- allocation-free FP64 solve ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 BiCGStab of isph_ppe_solve in the
+ * ISPH_PPE_FP64 form (same kernels, same reduction order, same restarts: the
+ * iterates equal those of isph_load_system + isph_ppe_solve bit for bit) on
+ * a system the caller holds in device memory, with no context.
+ *   n rows (0 <= n <= ISPH_NOALLOC_MAX_ROWS), ELL slot-major with leading
+ *   dimension ld >= n: entry k of row i is col[k*ld + i], val[k*ld + i] for
+ *   k < row_len[i] <= ISPH_NBR_CAP + 2; rhs[n]; x[n] holds the initial guess
+ *   on entry and the solution on return.  All five pointers are device
+ *   memory.  tol: relative residual target; max_iter, max_outer: iteration
+ *   and breakdown-restart caps (as isph_params).  stream: a hipStream_t, or
+ *   NULL for the null stream; the call returns after the stream has drained.
+ * The BiCGStab vectors, the block partials and the solver scalars live in a
+ * static device array of fixed size (6 * ISPH_NOALLOC_MAX_ROWS doubles plus
+ * ~64 KB, about 50 MB, held on every GPU the library runs on until the
+ * process ends); the call makes no hipMalloc, hipFree or other allocation.
+ * All calls on one GPU share that array, so calls that overlap in time on
+ * one GPU (from two host threads, or on two streams) race and must not be
+ * made.  Rows longer than ISPH_NBR_CAP + 2 or columns outside [0, n) are
+ * rejected (ISPH_EARG) by a device check before the first product; n above
+ * the maximum is rejected before any device call.  Errors are reported by
+ * isph_last_error(NULL).  Return codes as isph_ppe_solve. */
+#define ISPH_NOALLOC_MAX_ROWS (1 << 20)
+int isph_ppe_solve_noalloc(int32_t n, int32_t ld, const int32_t *row_len,
+                           const int32_t *col, const double *val, const double *rhs,
+                           double *x, double tol, int32_t max_iter, int32_t max_outer,
+                           void *stream, isph_solve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
