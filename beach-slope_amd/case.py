@@ -156,6 +156,64 @@ def _walls_paddle_flume(x0: float, x1: float, y0: float, ytop: float, dx: float,
             np.concatenate(fs))
 
 
+def kernel_porosity(params: dict, solids: dict, px: np.ndarray, py: np.ndarray) -> np.ndarray:
+    """Porosity at the points (px, py) from the solids' kernel sum, as effpor
+    computes it for a fluid particle there (porous.f90:52-100, 182-188):
+    n = 1 - min(sum over solids within 2h of (1 - por) W(r) m_s / rho_s, 1 - por)."""
+    from scipy.spatial import cKDTree
+
+    n = np.ones(px.size)
+    if solids["x"].size == 0 or px.size == 0:
+        return n
+    h, por = params["h"], params["por"]
+    pts = cKDTree(np.c_[px, py])
+    sol = cKDTree(np.c_[solids["x"], solids["y"]])
+    pairs = pts.sparse_distance_matrix(sol, 2 * h, output_type="coo_matrix")
+    i, j, r = pairs.row, pairs.col, pairs.data
+    keep = r < 2 * h
+    i, j, r = i[keep], j[keep], r[keep]
+    k1 = 10.0 / (float(np.float32(28.0) * np.float32(3.14)) * h * h)
+    k2 = 1.0 / (float(np.float32(3.0) * np.float32(3.14)) * h * h)
+    q = r / h
+    a = 2.0 - q
+    t1 = np.where(r < h, 4.0 - 6.0 * q * q + 3.0 * q ** 3, a ** 3)
+    t2 = np.where(r < h, q ** 3 - 6.0 * q + 6.0, a ** 3)
+    w = 0.1 * t1 * k1 + 0.9 * t2 * k2
+    sf = np.bincount(i, weights=(1.0 - por) * w * solids["m"][j] / solids["rho"][j],
+                     minlength=px.size)
+    return 1.0 - np.minimum(sf, 1.0 - por)
+
+
+def pore_volumes(c: Case) -> Case:
+    """Give every fluid particle the bulk volume of its lattice site,
+    m / rho = dx^2, where rho = n rho0 is the density effpor sets
+    (porous.f90:188-190): volume vol0 = n dx^2 and mass m = rho0 vol0, with n
+    the porosity at the particle.  A particle in the pores then represents
+    the pore water of one lattice site, and the particle volume m / rho that
+    every sum of the pipeline weights by is the lattice area on both sides
+    of the beach surface and at the walls.  The reference reaches the same
+    state differently: its pore water fills the medium on a lattice of
+    spacing 2 pr / sqrt(por) with the open-water mass (internal.f90:37-39,
+    setup.f90:71-72), so that m / rho = V0 / n is again the area per
+    particle.  Without it a pore particle would carry the open-water volume
+    V0 and m / rho = V0 / n would jump at the beach surface; comp_vel's
+    pressure term n_i sum_j V_j (p_i + p_j) gradW~ / rho0 then holds a
+    2 p_i sum_j V_j gradW~ that grows with depth, and at 0.3-0.4 m depth
+    the flume diverges within a few steps."""
+    fl = c.parts["tid"] == TID_FLUID
+    dx = c.meta["dx"]
+    n = kernel_porosity(c.params, c.solids, c.parts["x"][fl], c.parts["y"][fl])
+    vol0 = c.parts["vol0"].copy()
+    vol0[fl] = n * dx * dx
+    rho = c.parts["rho"].copy()
+    rho[fl] = n * c.parts["rho0"][fl]
+    c.parts["vol0"] = vol0
+    c.parts["m"] = c.parts["rho0"] * vol0
+    c.parts["rho"] = rho
+    c.meta["pore_volumes"] = True
+    return c
+
+
 def _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, depth_top, sx=None, sy=None,
               steps=1, meta=None, wflags=None) -> Case:
     n_f, n_w = fx.size, wx.size
@@ -230,15 +288,17 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
                  tank_h: float = 0.6, toe: float = 6.0, slope: float = 0.1,
                  porosity: float = 0.4, d50: float = 0.01,
                  patch=(0.2, 0.1), dt: float = 5.0e-5, steps: int = 2000,
-                 wave_h: float = 0.05, wave_t: float = 1.5, grade_h: float = 8.0,
+                 wave_h: float = 0.05, wave_t: float = 1.5, grade_h: float = 0.0,
                  wavemaker: bool = True, ramp: float | None = None,
-                 name: str = "porous_flume") -> Case:
+                 name: str = "porous_flume", pore_volume: bool = True) -> Case:
     """Wave-runup flume on a porous beach (the 650k-particle configuration at
-    the defaults: 4000 x 160 fluid lattice, 3 wall layers).  The 1:10 beach
-    under the line y = (x - toe)*slope is filled with static solid particles
-    at the fluid spacing; the fluid lattice fills the pores as in the
-    reference's seepage case.  A scalar patch of c = 1 (0.2 m x 0.1 m) sits at
-    the toe.
+    the defaults: 4000 x 160 fluid lattice, 3 wall layers, 128,000 beach
+    solids).  The 1:10 beach under the line y = (x - toe)*slope is filled
+    with static solid particles at the fluid spacing (bulk density 1800,
+    boundary.f90:523-593); the fluid lattice runs through the beach, and
+    `pore_volumes` gives each fluid particle the bulk volume of its lattice
+    site (m / rho = dx^2 at the porosity effpor computes there).  A scalar
+    patch of c = 1 (0.2 m x 0.1 m) sits at the toe.
 
     The left end is a piston wavemaker (with `wavemaker`): a three-layer
     paddle of F_PADDLE walls whose stroke gives waves of height wave_h and
@@ -251,12 +311,14 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
     `Scenario` and the oracle's wrapper apply on upload.  Without it the
     left end is an ordinary three-layer wall.
 
-    The beach surface is graded: solid volume ramps linearly from 0 at the
-    beach surface to full at a depth of grade_h*h, so the porosity goes from 1
-    to `porosity` over that layer.  With a sharp porosity jump the reference's
-    pressure projection amplifies the velocity divergence at the interface by
-    a dt-independent factor each step and the run diverges within a few
-    steps; a graded layer keeps it bounded."""
+    The beach surface is sharp by default: the layer has porosity `porosity`
+    right up to the slope line (the kernel sum smooths n over 2h).  With
+    grade_h > 0 the solid volume instead ramps linearly from 0 at the beach
+    surface to full at a depth of grade_h*h.
+
+    pore_volume=False leaves every fluid particle the open-water volume dx^2
+    (and mass rho0 dx^2), so m / rho = dx^2 / n jumps at the beach surface:
+    at 0.3-0.4 m depth that flume diverges within a few steps."""
     params = default_params(dx)
     params.update(dt=dt, por=porosity, d50=d50, tol=1.0e-10)
     wflags = None
@@ -301,35 +363,26 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
     c = _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, offy + depth, sx, sy,
                   steps=steps, meta=meta, wflags=wflags)
     c.solids["m"] = c.solids["m"] * weight
-    return c
+    return pore_volumes(c) if pore_volume else c
 
 
 def flume_650k(steps: int = 2000, dt: float = 5.0e-5) -> Case:
-    """The 650k-particle throughput workload: the 10 m x 0.4 m flume of
+    """The 650k-particle wave-runup workload: the 10 m x 0.4 m flume of
     `porous_flume` at dx = 0.0025 (4000 x 160 fluid lattice, 3 wall layers,
-    scalar patch at the toe), with the beach solids left out.  The porous
-    beach makes the reference's projection diverge within a few steps at this
-    depth (a volume jump m/rho = V0/n at the beach surface and at the walls
-    gives comp_vel a spurious 2 p_i sum_j V_j gradW term that grows with the
-    absolute pressure); without the solids the run stays bounded."""
-    c = porous_flume(steps=steps, dt=dt, name="flume_650k")
-    c.solids = {k: v[:0] for k, v in c.solids.items()}
-    c.meta["beach_solids"] = False
-    return c
+    piston wavemaker for H = 0.05 m, T = 1.5 s, scalar patch at the toe) on
+    its 1:10 porous beach of porosity 0.4 and d50 = 0.01 m (128,000 solids)."""
+    return porous_flume(steps=steps, dt=dt, name="flume_650k")
 
 
 def scenario_mix(n_scen: int = 64, seed: int = 2024, n_min: int = 20_000,
                  n_max: int = 300_000, log_uniform: bool = False, steps: int = 1000,
-                 beach_solids: bool = False):
-    """Independent flume scenarios of uneven particle count with parameters
-    drawn as the batched configuration states: slope 1:8-1:20, porosity
-    0.3-0.5, d50 0.005-0.02 m, H 0.02-0.08 m, T 1.0-2.5 s.  Each flume is
-    6 m x 0.3 m; the lattice spacing is chosen so the fluid count matches the
-    drawn N.  The porous beach solids are left out unless beach_solids is
-    set: at these depths the reference's projection diverges at the beach
-    surface within a few steps (see flume_650k), so the throughput mixes run
-    the flumes without them; the drawn porosity and d50 stay in the
-    parameters."""
+                 beach_solids: bool = True):
+    """Independent porous-beach flume scenarios of uneven particle count with
+    parameters drawn as the batched configuration states: slope 1:8-1:20,
+    porosity 0.3-0.5, d50 0.005-0.02 m, H 0.02-0.08 m, T 1.0-2.5 s.  Each
+    flume is 6 m x 0.3 m; the lattice spacing is chosen so the fluid count
+    matches the drawn N.  beach_solids=False leaves the beach solids out
+    (an impermeable-free flume with the drawn porosity and d50 unused)."""
     rng = np.random.default_rng(seed)
     cases = []
     for k in range(n_scen):
@@ -354,43 +407,6 @@ def scenario_mix(n_scen: int = 64, seed: int = 2024, n_min: int = 20_000,
             c.meta["beach_solids"] = False
         cases.append(c)
     return cases
-
-
-def match_wall_volumes(c: Case) -> Case:
-    """Give each wall particle the particle volume of the porous medium it sits
-    in: density n*rho with n from the solids' kernel sum (porous.f90:52-100,
-    182-190), so m/rho = V0/n matches the neighbouring fluid.  Walls never have
-    their density updated (effpor acts on fluid only), so this persists."""
-    from scipy.spatial import cKDTree
-
-    if c.ns == 0:
-        return c
-    P = c.params
-    h, por = P["h"], P["por"]
-    wall = c.parts["tid"] != TID_FLUID
-    wx, wy = c.parts["x"][wall], c.parts["y"][wall]
-    tree = cKDTree(np.c_[c.solids["x"], c.solids["y"]])
-    pairs = tree.query_ball_point(np.c_[wx, wy], r=2 * h)
-    k1 = 10.0 / (float(np.float32(28.0) * np.float32(3.14)) * h * h)
-    k2 = 1.0 / (float(np.float32(3.0) * np.float32(3.14)) * h * h)
-    sf = np.zeros(wx.size)
-    for i, js in enumerate(pairs):
-        if not js:
-            continue
-        js = np.asarray(js)
-        r = np.hypot(wx[i] - c.solids["x"][js], wy[i] - c.solids["y"][js])
-        q = r / h
-        a = 2.0 - q
-        t1 = np.where(r < h, 4.0 - 6.0 * q * q + 3.0 * q ** 3, a ** 3)
-        t2 = np.where(r < h, q ** 3 - 6.0 * q + 6.0, a ** 3)
-        w = 0.1 * t1 * k1 + 0.9 * t2 * k2
-        sf[i] = np.sum((1.0 - por) * w * c.solids["m"][js] / c.solids["rho"][js])
-    n = 1.0 - np.minimum(sf, 1.0 - por)
-    rho = c.parts["rho"].copy()
-    rho[wall] = rho[wall] * n
-    c.parts["rho"] = rho
-    c.meta["wall_volume_matched"] = True
-    return c
 
 
 def tiny_box(nx: int = 12, ny: int = 10, dx: float = 0.01, jitter: float = 0.05,

@@ -1,11 +1,13 @@
-"""The 653k-particle wave flume (1:10 porous beach, piston wavemaker, H 0.05 m,
-T 1.5 s, dx 2.5 mm) at full size, in the precision form.  The oracle takes
-minutes per step here, so the checks are properties that hold at any size:
-every solve meets the 1e-8 relative residual, particles and volumes are
-conserved, fixed walls stay put and the paddle moves rigidly by the piston's
-displacement, the counting sort leaves every particle in the cell range of its
-own cell, and on the same assembled system the precision form's pressure lies
-within 1e-6 (relative) of a FP64 solve converged to 1e-11."""
+"""The 653k-particle wave flume on its 1:10 porous beach (porosity 0.4,
+d50 0.01 m, 128,000 beach solids; piston wavemaker, H 0.05 m, T 1.5 s,
+dx 2.5 mm) at full size, in the plain FP64 form and in the precision form.
+The oracle takes minutes per step here, so the checks are properties that
+hold at any size: every solve meets the 1e-8 relative residual, particles
+and volumes are conserved, fixed walls stay put and the paddle moves rigidly
+by the piston's displacement, the counting sort leaves every particle in the
+cell range of its own cell, the beach's pore water keeps its porosity and
+volume m / rho = dx^2, and on the same assembled system the precision form's
+pressure lies within 1e-6 (relative) of a FP64 solve converged to 1e-11."""
 import numpy as np
 import pytest
 
@@ -17,9 +19,12 @@ def _order(s, mask):
     return idx[np.lexsort((s["x"][idx], s["y"][idx]))]
 
 
-def test_flume_650k_precision_form_properties(pkg, cases):
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "precision"])
+def test_flume_650k_properties(pkg, cases, precision):
     c = cases.flume_650k()
-    prm = dict(c.params, precision=1, tol=1e-8, max_outer=20)
+    assert c.ns == 128_000
+    print(f"flume_650k n={c.n} ns={c.ns} precision={precision}", flush=True)
+    prm = dict(c.params, precision=precision, tol=1e-8, max_outer=20)
     g = pkg.Scenario(c, params=prm)
     g.init_state()
     before = g.download()
@@ -43,6 +48,14 @@ def test_flume_650k_precision_form_properties(pkg, cases):
     assert np.array_equal(np.sort(before["m"][before["tid"] != 3]), np.sort(after["m"][~fl]))
     assert np.array_equal(np.bincount(before["tid"], minlength=4),
                           np.bincount(after["tid"], minlength=4))
+    # the beach: pore water particles keep their volume fraction, and the
+    # porosity effpor gives them keeps m / rho at the lattice area
+    pr = g.props()["porosity"]
+    dx = c.meta["dx"]
+    assert np.sum(pr[fl] < 0.5) > 100_000
+    inside = fl & (after["vol0"] < 0.99 * dx * dx)
+    assert inside.sum() > 100_000
+    assert np.all(pr[fl] >= c.params["por"] - 1e-12)
 
     # fixed walls keep their positions exactly; paddle walls move together
     fixed = lambda s: (s["tid"] != 3) & ((s["flags"] & cases.F_PADDLE) == 0)
@@ -76,6 +89,9 @@ def test_flume_650k_precision_form_properties(pkg, cases):
     assert np.all(y >= (c.params["celly"] - 2 - ci) * size - tol)
     assert np.all(y <= (c.params["celly"] - 1 - ci) * size + tol)
 
+    if not precision:
+        g.close()
+        return
     # the precision form's pressure against a tight FP64 solve of the same rows
     g.ppe_assemble()
     st32 = g.ppe_solve()
