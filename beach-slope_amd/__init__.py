@@ -26,6 +26,13 @@ LIB_PATH = HERE / "libisph_hip.so"
 NBR_CAP = 64
 ROW_CAP = NBR_CAP + 2
 NOALLOC_MAX_ROWS = 1 << 20   # ISPH_NOALLOC_MAX_ROWS
+# kernels the autotuner sizes (ISPH_TUNE_N, in isph_autotune's order)
+TUNED_KERNELS = ("neighbour", "effpor", "eddyvis", "int_vel_varts", "int_vel",
+                 "ppe_assemble", "comp_vel", "opt2_shift", "scalart")
+# launch shapes (isph_autotune): one thread per particle in blocks of 64, 128
+# or 256 threads; for the neighbour search also one workgroup per tile of 4
+# or 8 cells of a cell column, its neighbourhood staged in LDS
+TUNE_SHAPES = ("p64", "p128", "p256", "tile4", "tile8")
 PPE_FP64, PPE_FP32_REFINE = 0, 1
 
 ERRORS = {-1: "HIP error", -2: "bad argument", -3: "capacity overflow",
@@ -87,7 +94,9 @@ EXPORTS = ("isph_create", "isph_destroy", "isph_last_error", "isph_set_params",
            "isph_batch_destroy", "isph_set_buffer", "isph_buffer_update", "isph_set_wcsph",
            "isph_footprint_bytes", "isph_prof_enable", "isph_prof_read",
            "isph_wavemaker_motion", "isph_set_wavemaker", "isph_wavemaker",
-           "isph_ppe_solve_noalloc") + tuple("isph_" + p for p in PHASES)
+           "isph_ppe_solve_noalloc", "isph_batch_launches", "isph_batch_count",
+           "isph_batch_set_graph", "isph_autotune", "isph_batch_autotune", "isph_set_shape",
+           "isph_batch_set_shape") + tuple("isph_" + p for p in PHASES)
 
 _lib = None
 
@@ -133,6 +142,14 @@ def lib():
     L.isph_batch_step.argtypes = [vp, i32]
     L.isph_batch_stats.argtypes = [vp, i32, C.POINTER(SolveStats)]
     L.isph_batch_destroy.argtypes = [vp]
+    L.isph_batch_launches.restype = i64
+    L.isph_batch_launches.argtypes = [vp]
+    L.isph_batch_count.argtypes = [vp, i32, C.POINTER(i64)]
+    L.isph_batch_set_graph.argtypes = [vp, i32]
+    L.isph_autotune.argtypes = [vp, i32, vp, vp]
+    L.isph_batch_autotune.argtypes = [vp, i32, vp, vp]
+    L.isph_set_shape.argtypes = [vp, i32, i32]
+    L.isph_batch_set_shape.argtypes = [vp, i32, i32]
     L.isph_set_buffer.argtypes = [vp, i32, i64] + [vp] * 5 + [d] * 7
     L.isph_buffer_update.argtypes = [vp, i32]
     L.isph_set_wcsph.argtypes = [vp, d, d, d, d, i32]
@@ -215,9 +232,10 @@ class Scenario:
         return (lib().isph_last_error(self._ctx) or b"").decode()
 
     def set_params(self, **kw):
-        self.params.update(kw)
-        self._P = Params.from_dict(self.params)
-        self._check(lib().isph_set_params(self._ctx, C.byref(self._P)), "isph_set_params")
+        params = dict(self.params, **kw)
+        P = Params.from_dict(params)
+        self._check(lib().isph_set_params(self._ctx, C.byref(P)), "isph_set_params")
+        self.params, self._P = params, P
 
     # -- one method per reference subroutine ------------------------------
     def _phase(self, name):
@@ -364,6 +382,18 @@ class Scenario:
         self._check(lib().isph_load_system(self._ctx, len(rhs), _ptr(length), _ptr(col),
                                            _ptr(val), _ptr(rhs), _ptr(x0)), "isph_load_system")
 
+    def autotune(self, trials: int = 5) -> dict:
+        """Choose each particle kernel's launch shape by timing its shapes on
+        the current state (isph_autotune; call at a step boundary).
+        Returns {kernel: (chosen shape, {shape: mean us per launch})}."""
+        return _tune(lib().isph_autotune, self._ctx, trials, self._check, "isph_autotune")
+
+    def set_shape(self, kernel: str, shape: str):
+        """Launch `kernel` (one of TUNED_KERNELS) in `shape` (one of
+        TUNE_SHAPES) from now on (isph_set_shape)."""
+        self._check(lib().isph_set_shape(self._ctx, TUNED_KERNELS.index(kernel),
+                                         TUNE_SHAPES.index(shape)), "isph_set_shape")
+
     def prof_enable(self, on: bool = True):
         self._check(lib().isph_prof_enable(self._ctx, int(on)), "isph_prof_enable")
 
@@ -403,6 +433,38 @@ class Batch:
             raise IsphError(rc, "isph_batch_step", msgs)
         return rc
 
+    def set_shape(self, kernel: str, shape: str):
+        """isph_batch_set_shape: the launch shape of a batched particle phase."""
+        rc = lib().isph_batch_set_shape(self._b, TUNED_KERNELS.index(kernel),
+                                        TUNE_SHAPES.index(shape))
+        if rc != 0:
+            raise IsphError(rc, "isph_batch_set_shape", self.scenarios[0].last_error())
+
+    def autotune(self, trials: int = 3) -> dict:
+        """isph_batch_autotune: launch shapes of the batched particle phases."""
+        def check(rc, where):
+            if rc != 0:
+                raise IsphError(rc, where, self.scenarios[0].last_error())
+        return _tune(lib().isph_batch_autotune, self._b, trials, check, "isph_batch_autotune")
+
+    def set_graph(self, on: bool = True):
+        """Replay the batched phase sequences as HIP graphs (isph_batch_set_graph)."""
+        rc = lib().isph_batch_set_graph(self._b, int(on))
+        if rc != 0:
+            raise IsphError(rc, "isph_batch_set_graph", self.scenarios[0].last_error())
+
+    def launches(self) -> int:
+        """Kernel launches of the last batched step (0: phases per scenario)."""
+        return int(lib().isph_batch_launches(self._b))
+
+    def count(self, s: int) -> int:
+        """Live particles of scenario s after the last batched step."""
+        n = C.c_int64()
+        rc = lib().isph_batch_count(self._b, s, C.byref(n))
+        if rc != 0:
+            raise IsphError(rc, "isph_batch_count")
+        return int(n.value)
+
     def stats(self, s: int) -> SolveStats:
         st = SolveStats()
         rc = lib().isph_batch_stats(self._b, s, C.byref(st))
@@ -420,6 +482,16 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _tune(fn, handle, trials, check, where) -> dict:
+    shape = np.zeros(len(TUNED_KERNELS), np.int32)
+    nc = len(TUNE_SHAPES)
+    us = np.zeros(nc * len(TUNED_KERNELS))
+    check(fn(handle, int(trials), _ptr(shape), _ptr(us)), where)
+    return {k: (TUNE_SHAPES[shape[q]],
+                {t: v for t, v in zip(TUNE_SHAPES, us[nc * q:nc * q + nc].tolist()) if v >= 0})
+            for q, k in enumerate(TUNED_KERNELS)}
 
 
 def wavemaker_motion(stroke: float, period: float, ramp: float, t0: float, t: float):

@@ -13,6 +13,14 @@ the 16-bit offset j - i in the precision form) and the FP64 value.
   kb_pb_1   t1 = A ph, partial (t1 . rh32)    reads alen, ph (FP32), rh32; writes t1 (FP64)
   kb_pb_2   t2 = A sh, partials (t2 . s), (t2 . t2)
                                              reads alen, sh (FP32), s (FP64); writes t2 (FP64)
+  kb_mg_pre0   r0 = v - A16 x0 (level 0 of the V-cycle, FP16 values, 16-bit
+               offsets)                      reads alen, v (FP32 or FP64: 6 on average), x0 (own);
+                                             writes r0
+  kb_mg_post0  x = x1 + w (v - A16 x1) d     reads alen, v, x1, d0; writes x
+  kb_mg_coarse the coarse cell levels in one workgroup: reads their stencils
+               (9 FP32) and 1/diagonal, the first level's right-hand side
+               and the dense FP64 inverse; writes the first level's result
+               (coarse_bytes)
 """
 from __future__ import annotations
 
@@ -21,8 +29,36 @@ _PER_ROW = {
     "k_bicg_2": (4 + 8 + 8, 16),
     "kb_pb_1": (4 + 4 + 4, 8),
     "kb_pb_2": (4 + 4 + 8, 8),
+    "kb_mg_pre0": (4 + 6 + 4, 4),
+    "kb_mg_post0": (4 + 6 + 4 + 4, 4),
 }
-_PER_NNZ = {"k_bicg_1": 12, "k_bicg_2": 12, "kb_pb_1": 10, "kb_pb_2": 10}
+_PER_NNZ = {"k_bicg_1": 12, "k_bicg_2": 12, "kb_pb_1": 10, "kb_pb_2": 10,
+            "kb_mg_pre0": 4, "kb_mg_post0": 4}
+
+MG_DENSE_MAX = 64
+MG_CO_UNKNOWNS = 1024   # unknowns of the coarse workgroup's first level, at most
+
+
+def cell_levels(cellx: int, celly: int) -> list[int]:
+    """Unknowns of the multigrid cell levels 1.. (2x2 aggregation down to at
+    most MG_DENSE_MAX unknowns)."""
+    ny, nx, out = celly, cellx, []
+    while True:
+        out.append(ny * nx)
+        if ny * nx <= MG_DENSE_MAX:
+            return out
+        ny, nx = (ny + 1) // 2, (nx + 1) // 2
+
+
+def coarse_bytes(cellx: int, celly: int) -> float:
+    """Algorithmic bytes of one kb_mg_coarse launch: from the first level
+    with at most MG_CO_UNKNOWNS unknowns to the dense one, the stencils and
+    1/diagonal of every level but the dense one, the first level's
+    right-hand side and result, and the dense FP64 inverse."""
+    lv = cell_levels(cellx, celly)
+    k = next(i for i, n in enumerate(lv) if n <= MG_CO_UNKNOWNS)
+    tail = lv[k:]
+    return 4.0 * 10 * sum(tail[:-1]) + 4.0 * 2 * tail[0] + 8.0 * tail[-1] ** 2
 
 
 def kernel_bytes(kernel: str, n: int, nnz: float) -> float:

@@ -246,11 +246,12 @@ def _assemble(params, name, fx, fy, fc, wx, wy, wt, dx, depth_top, sx=None, sy=N
 
 def dam_break(dx: float = 0.005, jitter: float = 0.0, seed: int = 1234,
               tank=(2.0, 0.6), column=(0.5, 0.25), toe: float = 1.0,
-              slope: float = 0.1, steps: int = 200) -> Case:
+              slope: float = 0.1, steps: int = 200, h_fac: float = 1.3) -> Case:
     """Dam break onto a 1:10 impermeable slope (FP64 configuration, and with a
     seeded uniform +-jitter*dx on fluid positions the precision-form one).
-    Tank 2.0 x 0.6 m, water column 0.5 x 0.25 m, h = 1.3 dx, dt = 1e-4 s."""
-    params = default_params(dx)
+    Tank 2.0 x 0.6 m, water column 0.5 x 0.25 m, h = 1.3 dx (h_fac dx),
+    dt = 1e-4 s."""
+    params = default_params(dx, h_fac)
     params.update(dt=1.0e-4, tol=1.0e-10)
     off = 3.0 * dx + 0.5 * dx            # keeps the walls at x, y >= 0
     L, H = tank

@@ -44,7 +44,7 @@ struct DevParams {
 // umax, numax of initialize.f90:21-60) plus solver state.
 struct DevScalars {
   double t, dt, dtsol, umax, numax;
-  unsigned long long umax_bits, numax_bits;  // pending maxima from comp_vel
+  unsigned long long umax_bits, numax_bits;  // unused (layout kept)
   int solsteps, n, n_hi_unused, err;
   // BiCGStab state
   double rho, alpha, omega, beta, rr, bnorm, stop, rel;
@@ -64,8 +64,13 @@ struct Dev {
   DevScalars* sc;
   int cap;
   int nblk;  // row blocks of RED_BLOCK rows covering the host's upper bound of n
-  // per row block: running maximum of |velocity| (bits of a non-negative double)
-  // at [b] and of |nu_t| at [nmaxb + b], folded into the scalars by timestep
+  // per cell: the maxima of |velocity| and |nu_t| over the cell's particles
+  // at the last comp_vel in which the cell held particles (the reference's
+  // dpcell%maxvel / maxeddy, integrator.f90:831-841), as bits of
+  // non-negative doubles; an emptied cell keeps its values
+  unsigned long long *cmaxv, *cmaxn;
+  // per block of 256 cells: the maxima over its cells, at [b] and [nmaxb + b],
+  // folded by timestep (integrator.f90:914-924)
   unsigned long long* maxb;
   int nmaxb;
   // gathered particle state (ping-pong buffers swapped by the host)

@@ -26,14 +26,17 @@ __global__ void k_fill_i32(int32_t* a, int len, int val) {
 // ---------------------------------------------------------------------
 // projection, isph.f90:17-47
 // ---------------------------------------------------------------------
-__global__ void k_projection(Dev D) {
-  PIDX();
+__device__ __forceinline__ void projection_body(const Dev& D, const int i, const int n) {
   if (i >= n || D.tid[i] != TID_FLUID) return;
   const double dt = D.sc->dt;
   D.xs[i] = D.x[i];
   D.x[i] = D.x[i] + ((dt * D.u[i]) / D.por[i]);
   D.ys[i] = D.y[i];
   D.y[i] = D.y[i] + ((dt * D.v[i]) / D.por[i]);
+}
+__global__ void k_projection(Dev D) {
+  PIDX();
+  projection_body(D, i, n);
 }
 
 // ---------------------------------------------------------------------
@@ -42,8 +45,7 @@ __global__ void k_projection(Dev D) {
 // The PPE source term reads a wall neighbour's intermediate velocity (us, vs),
 // which int_vel leaves to walls, so the paddle sets it too.
 // ---------------------------------------------------------------------
-__global__ void k_paddle(Dev D, double X, double U) {
-  PIDX();
+__device__ __forceinline__ void paddle_body(const Dev& D, const int i, const int n, double X, double U) {
   if (i >= n || D.tid[i] == TID_FLUID || !(D.flags[i] & F_PADDLE)) return;
   D.x[i] = D.xs[i] + X;
   D.u[i] = U;
@@ -51,10 +53,17 @@ __global__ void k_paddle(Dev D, double X, double U) {
   D.us[i] = U;
   D.vs[i] = 0.0;
 }
-__global__ void k_paddle_rest(Dev D, double X) {
+__global__ void k_paddle(Dev D, double X, double U) {
   PIDX();
+  paddle_body(D, i, n, X, U);
+}
+__device__ __forceinline__ void paddle_rest_body(const Dev& D, const int i, const int n, double X) {
   if (i >= n || D.tid[i] == TID_FLUID || !(D.flags[i] & F_PADDLE)) return;
   D.xs[i] = D.x[i] - X;
+}
+__global__ void k_paddle_rest(Dev D, double X) {
+  PIDX();
+  paddle_rest_body(D, i, n, X);
 }
 
 // ---------------------------------------------------------------------
@@ -137,8 +146,7 @@ __device__ __forceinline__ int rebin_bin(const Dev& D, int i, int alt) {
 // The lanes of a wavefront hold consecutive particles, so a run of lanes with
 // one bin (a cell's stayers, mostly) adds its length with one atomic from its
 // first lane.  The counts are the same integers as one atomic per particle.
-__global__ void k_rebin_classify(Dev D, int alt) {
-  PIDX();
+__device__ __forceinline__ void rebin_classify_body(const Dev& D, const int i, const int n, int alt) {
   const int bin = (i < n) ? rebin_bin(D, i, alt) : -2;
   if (i < n) D.pbin[i] = bin;
   const int lane = threadIdx.x & 63;
@@ -149,11 +157,14 @@ __global__ void k_rebin_classify(Dev D, int alt) {
   const int len = later ? __ffsll((long long)later) : 64 - lane;
   if (head && bin >= 0) atomicAdd(&D.bincnt[bin], len);
 }
+__global__ void k_rebin_classify(Dev D, int alt) {
+  PIDX();
+  rebin_classify_body(D, i, n, alt);
+}
 
 // rank of a particle among the earlier particles of its source cell with the
 // same bin (all particles of one bin come from one source cell)
-__global__ void k_rebin_rank(Dev D) {
-  PIDX();
+__device__ __forceinline__ void rebin_rank_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   const int bin = D.pbin[i];
   if (bin < 0) return;
@@ -168,10 +179,13 @@ __global__ void k_rebin_rank(Dev D) {
   }
   D.prank[i] = rank;
 }
+__global__ void k_rebin_rank(Dev D) {
+  PIDX();
+  rebin_rank_body(D, i, n);
+}
 
 // scatter every gathered field into the new order (ping-pong buffers)
-__global__ void k_rebin_scatter(Dev D) {
-  PIDX();
+__device__ __forceinline__ void rebin_scatter_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   const int bin = D.pbin[i];
   if (bin < 0) return;
@@ -183,11 +197,17 @@ __global__ void k_rebin_scatter(Dev D) {
   D.tid2[d] = D.tid[i]; D.flags2[d] = D.flags[i]; D.vic2[d] = D.vic[i];
   D.cell2[d] = bin / NCLS;
 }
+__global__ void k_rebin_scatter(Dev D) {
+  PIDX();
+  rebin_scatter_body(D, i, n);
+}
 
-__global__ void k_rebin_finish(Dev D, int ncell) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rebin_finish_body(const Dev& D, int c, int ncell) {
   if (c <= ncell) D.cstart[c] = D.binstart[c * NCLS];
   if (c == 0) D.sc->n = D.binstart[ncell * NCLS];
+}
+__global__ void k_rebin_finish(Dev D, int ncell) {
+  rebin_finish_body(D, blockIdx.x * blockDim.x + threadIdx.x, ncell);
 }
 
 // ---------------------------------------------------------------------
@@ -195,10 +215,11 @@ __global__ void k_rebin_finish(Dev D, int ncell) {
 // ---------------------------------------------------------------------
 #define SCAN_ITEMS 4
 #define SCAN_TILE (256 * SCAN_ITEMS)
-__global__ void k_scan_tiles(const int32_t* in, int32_t* out, int32_t* tile_sums, int len) {
+__device__ __forceinline__ void scan_tile_body(const int32_t* in, int32_t* out,
+                                               int32_t* tile_sums, int len, int tile) {
   __shared__ int32_t s[256];
   const int t = threadIdx.x;
-  const int base = blockIdx.x * SCAN_TILE + t * SCAN_ITEMS;
+  const int base = tile * SCAN_TILE + t * SCAN_ITEMS;
   int32_t v[SCAN_ITEMS], sum = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++) {
@@ -219,10 +240,13 @@ __global__ void k_scan_tiles(const int32_t* in, int32_t* out, int32_t* tile_sums
     if (base + k < len) out[base + k] = run;
     run += v[k];
   }
-  if (t == 255) tile_sums[blockIdx.x] = s[255];
+  if (t == 255) tile_sums[tile] = s[255];
+}
+__global__ void k_scan_tiles(const int32_t* in, int32_t* out, int32_t* tile_sums, int len) {
+  scan_tile_body(in, out, tile_sums, len, blockIdx.x);
 }
 
-__global__ void k_scan_sums(int32_t* tile_sums, int ntiles, int32_t* total_out) {
+__device__ __forceinline__ void scan_sums_body(int32_t* tile_sums, int ntiles, int32_t* total_out) {
   __shared__ int32_t s[256];
   const int t = threadIdx.x;
   int32_t carry = 0;
@@ -243,6 +267,9 @@ __global__ void k_scan_sums(int32_t* tile_sums, int ntiles, int32_t* total_out) 
   }
   if (t == 0 && total_out) *total_out = carry;
 }
+__global__ void k_scan_sums(int32_t* tile_sums, int ntiles, int32_t* total_out) {
+  scan_sums_body(tile_sums, ntiles, total_out);
+}
 
 __global__ void k_scan_add(int32_t* out, const int32_t* tile_sums, int len) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,11 +279,15 @@ __global__ void k_scan_add(int32_t* out, const int32_t* tile_sums, int len) {
 // ---------------------------------------------------------------------
 // neighbour / nsearch2, search.f90:13-399
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_neighbour(Dev D) {
-  PIDX();
-  if (i >= n) return;
+// The search for particle i.  POS(j, x, y) gives particle j's position,
+// MR(j, m, rho) its mass and density: read from global memory by the
+// per-particle kernel, from the workgroup's LDS tile by the tiled one; the
+// arithmetic and its order are the same, so both give the same bits.
+template <typename POS, typename MR>
+__device__ __forceinline__ void neighbour_core(const Dev& D, const int i, POS pos, MR mr) {
   const DevParams& P = D.P;
-  const double xi = D.x[i], yi = D.y[i];
+  double xi, yi;
+  pos(i, xi, yi);
   int ci, cj;
   double xl, xr, yb, yt;
   cell_bounds(D.cell[i], P, ci, cj, xl, xr, yb, yt);
@@ -284,7 +315,7 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
 #pragma unroll
       for (int u = 0; u < NB_CAND; u++) {
         const int j = (b0 + u < je) ? b0 + u : b0;
-        cx[u] = D.x[j]; cy[u] = D.y[j];
+        pos(j, cx[u], cy[u]);
       }
 #pragma unroll
       for (int u = 0; u < NB_CAND; u++) {
@@ -314,7 +345,8 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
 #pragma unroll
     for (int q = 0; q < NB_PAIR; q++) {
       const int j = jn[q];
-      xb[q] = D.x[j]; yb[q] = D.y[j]; mb[q] = D.m[j]; rb[q] = D.rho[j];
+      pos(j, xb[q], yb[q]);
+      mr(j, mb[q], rb[q]);
     }
 #pragma unroll
     for (int q = 0; q < NB_PAIR; q++) {
@@ -355,11 +387,101 @@ __global__ void __launch_bounds__(256) k_neighbour(Dev D) {
   D.flags[i] = fl;
 }
 
+__device__ __forceinline__ void neighbour_body(const Dev& D, const int i, const int n) {
+  if (i >= n) return;
+  neighbour_core(
+      D, i, [&](int j, double& x, double& y) { x = D.x[j]; y = D.y[j]; },
+      [&](int j, double& m, double& r) { m = D.m[j]; r = D.rho[j]; });
+}
+__global__ void __launch_bounds__(256) k_neighbour(Dev D) {
+  PIDX();
+  neighbour_body(D, i, n);
+}
+
+// ---------------------------------------------------------------------
+// The neighbour search with its cells staged in LDS.  A workgroup of
+// 32 ROWS threads takes a tile of ROWS cells of one cell column.  A particle
+// of cell (row ci, column cj) searches the 2 x 2 cells of its quadrant:
+// columns cj-1..cj+1 and rows ci-1..ci+1 hold every candidate, so the tile's
+// particles find theirs in the three columns around it, rows one below to
+// one above the tile, and each column's part of that is one contiguous range
+// of particle indices (cell-sorted order).  The workgroup loads the three
+// ranges' positions, masses and densities into LDS with coalesced loads,
+// then each thread searches for particles of the tile reading LDS only.  A
+// tile whose neighbourhood exceeds CAP particles, or that holds a particle in
+// an edge cell of the grid, searches from global memory instead.  Lists, sums and flags are
+// those of the per-particle kernel, bit for bit (the same core, other
+// loaders).
+// ---------------------------------------------------------------------
+__host__ __device__ inline int ntile_count(int rows, int cellx, int celly) {
+  return cellx * ((celly + rows - 1) / rows);
+}
+
+template <int ROWS, int CAP>
+__device__ __forceinline__ void neighbour_tile(const Dev& D, const int tile) {
+  __shared__ double lx[CAP], ly[CAP], lm[CAP], lr[CAP];
+  const int cellx = D.P.cellx, celly = D.P.celly;
+  const int tpc = (celly + ROWS - 1) / ROWS;  // tiles per column
+  const int cj = tile / tpc, r0 = (tile - cj * tpc) * ROWS;
+  if (cj >= cellx) return;                   // the whole workgroup
+  const int r1 = min(r0 + ROWS, celly);      // tile rows [r0, r1)
+  const int p0 = D.cstart[cj * celly + r0], p1 = D.cstart[cj * celly + r1];
+  if (p1 <= p0) return;
+  const int n = D.sc->n;
+  // a particle in an edge cell of the grid would search past it: such tiles
+  // search from global memory
+  const int k = cj * celly;
+  if (cj < 1 || cj + 1 >= cellx || (r0 == 0 && D.cstart[k + 1] > D.cstart[k]) ||
+      (r1 == celly && D.cstart[k + celly] > D.cstart[k + celly - 1])) {
+    for (int i = p0 + (int)threadIdx.x; i < p1; i += blockDim.x) neighbour_body(D, i, n);
+    return;
+  }
+  // the three columns' ranges over rows r0-1 .. r1 (within the grid)
+  const int lo = max(r0 - 1, 0), hi = min(r1 + 1, celly);
+  int gs[3], ge[3], off[3];
+  int tot = 0;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int kq = (cj - 1 + q) * celly;
+    gs[q] = D.cstart[kq + lo];
+    ge[q] = D.cstart[kq + hi];
+    off[q] = tot;
+    tot += ge[q] - gs[q];
+  }
+  if (tot > CAP) {
+    for (int i = p0 + (int)threadIdx.x; i < p1; i += blockDim.x) neighbour_body(D, i, n);
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    for (int t = threadIdx.x; t < ge[q] - gs[q]; t += blockDim.x) {
+      const int j = gs[q] + t, l = off[q] + t;
+      lx[l] = D.x[j]; ly[l] = D.y[j]; lm[l] = D.m[j]; lr[l] = D.rho[j];
+    }
+  __syncthreads();
+  // particle index -> LDS slot (the ranges are ordered by column)
+  auto slot = [&](int j) {
+    return j >= gs[2] ? off[2] + (j - gs[2]) : (j >= gs[1] ? off[1] + (j - gs[1]) : j - gs[0]);
+  };
+  for (int i = p0 + (int)threadIdx.x; i < p1; i += blockDim.x)
+    neighbour_core(
+        D, i, [&](int j, double& x, double& y) { const int l = slot(j); x = lx[l]; y = ly[l]; },
+        [&](int j, double& m, double& r) { const int l = slot(j); m = lm[l]; r = lr[l]; });
+}
+// the two tile shapes: 4 rows on 128 threads, 8 rows on 256
+#define NT_CAP4 640
+#define NT_CAP8 1152
+__global__ void __launch_bounds__(128) k_neighbour_tile4(Dev D) {
+  neighbour_tile<4, NT_CAP4>(D, blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_neighbour_tile8(Dev D) {
+  neighbour_tile<8, NT_CAP8>(D, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------
 // effpor + porsearch, porous.f90:32-224
 // ---------------------------------------------------------------------
-__global__ void k_effpor(Dev D) {
-  PIDX();
+__device__ __forceinline__ void effpor_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   const DevParams& P = D.P;
   D.resist[i] = 0.0;
@@ -402,12 +524,15 @@ __global__ void k_effpor(Dev D) {
   const double b = (((fch * sp) * sqrt(P.lamk)) * (1.0 - nn)) / (sqrt(n3) * P.d50);
   D.resist[i] = a - b;
 }
+__global__ void k_effpor(Dev D) {
+  PIDX();
+  effpor_body(D, i, n);
+}
 
 // ---------------------------------------------------------------------
 // eddyvis, turbulence.f90:12-110
 // ---------------------------------------------------------------------
-__global__ void k_eddyvis(Dev D) {
-  PIDX();
+__device__ __forceinline__ void eddyvis_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   if (D.tid[i] != TID_FLUID) {
     D.nut[i] = 0.0;
@@ -450,12 +575,15 @@ __global__ void k_eddyvis(Dev D) {
   D.nut[i] = ((0.150 * dl) * (0.150 * dl)) * t4;
   D.tke[i] = (0.080 / 1.0) * ((dl * t4) * (dl * t4));
 }
+__global__ void k_eddyvis(Dev D) {
+  PIDX();
+  eddyvis_body(D, i, n);
+}
 
 // ---------------------------------------------------------------------
 // int_vel, integrator.f90:459-703 (pass 1: varts of free particles)
 // ---------------------------------------------------------------------
-__global__ void k_int_vel_varts(Dev D) {
-  PIDX();
+__device__ __forceinline__ void int_vel_varts_body(const Dev& D, const int i, const int n) {
   if (i >= n || !(D.flags[i] & F_FREE)) return;
   const DevParams& P = D.P;
   const int cap = D.cap, cnt = D.ncnt[i];
@@ -475,10 +603,13 @@ __global__ void k_int_vel_varts(Dev D) {
   D.varts[i] = (((1.0 * (P.dl * P.dl)) * sqrt((t1 * t1) + (t2 * t2))) * sqrt((uu * uu) + (vv * vv))) /
                ((D.sc->umax + (double)1e-6f) * D.por[i]);
 }
+__global__ void k_int_vel_varts(Dev D) {
+  PIDX();
+  int_vel_varts_body(D, i, n);
+}
 
 // pass 2: intermediate velocity (viscous + SPS + tke terms, Darcy-Forchheimer)
-__global__ void __launch_bounds__(256) k_int_vel(Dev D) {
-  PIDX();
+__device__ __forceinline__ void int_vel_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   const DevParams& P = D.P;
   const double dt = D.sc->dt;
@@ -567,6 +698,10 @@ __global__ void __launch_bounds__(256) k_int_vel(Dev D) {
     D.us[i] = D.u[i];
     D.vs[i] = (P.g * ni) * dt;
   }
+}
+__global__ void __launch_bounds__(256) k_int_vel(Dev D) {
+  PIDX();
+  int_vel_body(D, i, n);
 }
 
 // ---------------------------------------------------------------------
@@ -686,6 +821,10 @@ __device__ __forceinline__ int ppe_assemble_row(const Dev& D, int i) {
     dpos = L;
     L += 1 + ((tidi == TID_GHOST && cnt > 0 && !first_written) ? 1 : 0);
   }
+  // A particle with no neighbours gets the identity row, so p = 0 (its rhs
+  // is 0): the diagonal value 1 the reference leaves commented out at
+  // isph.f90:425-427, where its live code divides 0 by 0 (isph.f90:432-440).
+  if (cnt == 0) diag = 1.0;
   const double t1 = diag;
   if (!(isfinite(t1) && t1 != 0.0)) atomicOr(&D.sc->err, ERR_DIAG);
   D.acol[dpos * cap + i] = i;
@@ -706,11 +845,14 @@ __device__ __forceinline__ int ppe_assemble_row(const Dev& D, int i) {
 
 // the stored entries are counted per workgroup: one atomic per workgroup
 // rather than one per row on a single address
-__global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
-  PIDX();
+__device__ __forceinline__ void ppe_assemble_body(const Dev& D, const int i, const int n) {
   const int L = (i < n) ? ppe_assemble_row(D, i) : 0;
   const unsigned long long s = block_sum_u64((unsigned long long)L);
   if (threadIdx.x == 0 && s != 0) atomicAdd(&D.sc->nnz, s);
+}
+__global__ void __launch_bounds__(256) k_ppe_assemble(Dev D) {
+  PIDX();
+  ppe_assemble_body(D, i, n);
 }
 
 // ---------------------------------------------------------------------
@@ -722,22 +864,7 @@ __device__ __forceinline__ unsigned long long max_bits(double v) {
   return (a == a) ? (unsigned long long)__double_as_longlong(a) : 0ull;
 }
 
-// the row block's maxima added to its running maxima (the block owns its
-// slot, so no atomic is needed); every thread of the block must call it
-__device__ __forceinline__ void block_running_max(const Dev& D, unsigned long long vb,
-                                                  unsigned long long nb) {
-  block_max2_u64(vb, nb);
-  if (threadIdx.x == 0) {
-    unsigned long long* sv = D.maxb + blockIdx.x;
-    unsigned long long* sn = D.maxb + D.nmaxb + blockIdx.x;
-    const unsigned long long ov = *sv, on = *sn;
-    *sv = (vb > ov) ? vb : ov;
-    *sn = (nb > on) ? nb : on;
-  }
-}
-
-__device__ __forceinline__ void comp_vel_row(const Dev& D, int i, unsigned long long& vbits,
-                                             unsigned long long& nbits) {
+__device__ __forceinline__ void comp_vel_row(const Dev& D, int i) {
   const DevParams& P = D.P;
   const double dt = D.sc->dt;
   const bool buf = (D.flags[i] & F_BUFFER) != 0;
@@ -788,29 +915,68 @@ __device__ __forceinline__ void comp_vel_row(const Dev& D, int i, unsigned long 
     v = D.vs[i] + v;
     D.u[i] = u;
     D.v[i] = v;
-    const double vel = sqrt((u * u) + (v * v)) / ni;
-    D.vel[i] = vel;
-    vbits = max_bits(vel);
+    D.vel[i] = sqrt((u * u) + (v * v)) / ni;
   }
   if (buf) {
     D.u[i] = D.us[i];
     D.v[i] = 0.0;
   }
-  nbits = max_bits(D.nut[i]);
 }
 
-// the running maxima of |velocity| and |nu_t| go to per-block slots (one
-// atomic per particle on a single address took half of this kernel's time)
+__device__ __forceinline__ void comp_vel_body(const Dev& D, const int i, const int n) {
+  if (i < n) comp_vel_row(D, i);
+}
 __global__ void __launch_bounds__(256) k_comp_vel(Dev D) {
   PIDX();
-  unsigned long long vbits = 0, nbits = 0;
-  if (i < n) comp_vel_row(D, i, vbits, nbits);
-  block_running_max(D, vbits, nbits);
+  comp_vel_body(D, i, n);
+}
+
+// The per-cell maxima of comp_vel (integrator.f90:831-841): a cell that
+// holds particles takes the maxima of |vel| and |nu_t| over them; an empty
+// cell keeps the values of the last comp_vel that found particles in it.
+// vel is that of the non-buffer fluid particles just updated (walls
+// contribute 0, as their never-written vel does in the reference; so does a
+// buffer particle, whose slot the reference reads stale).  With init the
+// velocities count as 0 and empty cells are cleared: the state before the
+// first step.  Each block of 256 cells leaves the maxima over its cells for
+// timestep (one thread per cell; non-negative doubles compare as their bits).
+__device__ __forceinline__ void cell_max_cell(const Dev& D, int k, int ncell, int init,
+                                              int blk) {
+  unsigned long long vb = 0ull, nb = 0ull;
+  if (k < ncell) {
+    const int i0 = D.cstart[k], i1 = D.cstart[k + 1];
+    if (i1 > i0) {
+      for (int i = i0; i < i1; i++) {
+        if (!init && D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER)) {
+          const unsigned long long b = max_bits(D.vel[i]);
+          vb = (b > vb) ? b : vb;
+        }
+        const unsigned long long b = max_bits(D.nut[i]);
+        nb = (b > nb) ? b : nb;
+      }
+      D.cmaxv[k] = vb;
+      D.cmaxn[k] = nb;
+    } else if (init) {
+      D.cmaxv[k] = 0ull;
+      D.cmaxn[k] = 0ull;
+    } else {
+      vb = D.cmaxv[k];
+      nb = D.cmaxn[k];
+    }
+  }
+  block_max2_u64(vb, nb);
+  if (threadIdx.x == 0) {
+    D.maxb[blk] = vb;
+    D.maxb[D.nmaxb + blk] = nb;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cell_max(Dev D, int ncell, int init) {
+  cell_max_cell(D, blockIdx.x * blockDim.x + threadIdx.x, ncell, init, blockIdx.x);
 }
 
 // comp_pos, integrator.f90:845-883
-__global__ void k_comp_pos(Dev D) {
-  PIDX();
+__device__ __forceinline__ void comp_pos_body(const Dev& D, const int i, const int n) {
   if (i >= n || D.tid[i] != TID_FLUID) return;
   const double dt = D.sc->dt;
   const double x = ((D.x[i] + D.xs[i]) + ((dt * D.u[i]) / D.por[i])) / 2.0;
@@ -820,16 +986,23 @@ __global__ void k_comp_pos(Dev D) {
   if ((D.flags[i] & F_BUFFER) && (x > D.P.buf_xmin) && (x < D.P.buf_xmax))
     D.flags[i] = D.flags[i] & (uint8_t)~F_BUFFER;
 }
+__global__ void k_comp_pos(Dev D) {
+  PIDX();
+  comp_pos_body(D, i, n);
+}
 
 // freesurf, isph.f90:49-86: surface particles mark their neighbours
 // (plain byte stores of the same value; the order of the stores is irrelevant)
-__global__ void k_freesurf(Dev D) {
-  PIDX();
+__device__ __forceinline__ void freesurf_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   if (D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER) && D.gradvx[i] <= D.P.fs_thresh) {
     const int cap = D.cap, cnt = D.ncnt[i];
     for (int k = 0; k < cnt; k++) D.vic[D.nbr[k * cap + i]] = 1;
   }
+}
+__global__ void k_freesurf(Dev D) {
+  PIDX();
+  freesurf_body(D, i, n);
 }
 
 // ---------------------------------------------------------------------
@@ -838,12 +1011,21 @@ __global__ void k_freesurf(Dev D) {
 // -csh*h^2*frac*gradW*V; the stored shift is the capped one, the velocity
 // correction uses the uncapped shift.
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_opt2_shift(Dev D) {
-  PIDX();
+__device__ __forceinline__ void opt2_shift_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   if (!(D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER))) return;
   const int cnt = D.ncnt[i];
-  if (cnt == 0) return;
+  if (cnt == 0) {
+    // no neighbour: no shift.  The reference zeroes its local shifts
+    // (part_shift.f90:32-35) but stores them only for count > 0
+    // (part_shift.f90:38-161), so its apply step (part_shift.f90:179-182)
+    // would add the slot's saved position and intermediate velocity.
+    D.xs[i] = 0.0;
+    D.ys[i] = 0.0;
+    D.us[i] = 0.0;
+    D.vs[i] = 0.0;
+    return;
+  }
   const DevParams& P = D.P;
   const int cap = D.cap;
   const double c1 = D.cf1[i], c2 = D.cf2[i], c3 = D.cf3[i], c4 = D.cf4[i];
@@ -887,9 +1069,12 @@ __global__ void __launch_bounds__(256) k_opt2_shift(Dev D) {
   D.us[i] = a1 * xsa + a2 * ysa;
   D.vs[i] = b1 * xsa + b2 * ysa;
 }
-
-__global__ void k_shift_apply(Dev D) {
+__global__ void __launch_bounds__(256) k_opt2_shift(Dev D) {
   PIDX();
+  opt2_shift_body(D, i, n);
+}
+
+__device__ __forceinline__ void shift_apply_body(const Dev& D, const int i, const int n) {
   if (i >= n) return;
   if (D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER)) {
     D.x[i] = D.x[i] + D.xs[i];
@@ -898,10 +1083,13 @@ __global__ void k_shift_apply(Dev D) {
     D.v[i] = D.v[i] + D.vs[i];
   }
 }
+__global__ void k_shift_apply(Dev D) {
+  PIDX();
+  shift_apply_body(D, i, n);
+}
 
 // densityupdate, scalar.f90:399-426
-__global__ void k_densityupdate(Dev D) {
-  PIDX();
+__device__ __forceinline__ void densityupdate_body(const Dev& D, const int i, const int n) {
   if (i >= n || D.tid[i] != TID_FLUID) return;
   const double rhomax = D.P.rho * D.P.rel_den, rhomin = D.P.rhomin;
   const double r0 = (0.5 * (rhomax + rhomin)) + (D.c[i] * (rhomax - rhomin));
@@ -909,12 +1097,15 @@ __global__ void k_densityupdate(Dev D) {
   D.m[i] = r0 * D.vol0[i];
   D.rho[i] = r0 * D.por[i];
 }
+__global__ void k_densityupdate(Dev D) {
+  PIDX();
+  densityupdate_body(D, i, n);
+}
 
-// timestep, integrator.f90:885-953 (one workgroup: the row blocks' running
-// maxima are folded into the scalars and cleared, then one thread)
+// timestep, integrator.f90:885-953 (one workgroup: the maxima over every
+// cell, from the cell blocks' maxima, then one thread)
 __device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
-__global__ void __launch_bounds__(256) k_timestep(Dev D) {
-  if (blockIdx.x != 0) return;
+__device__ __forceinline__ void timestep_block(const Dev& D) {
   unsigned long long mv = 0, mn = 0;
   for (int b0 = threadIdx.x; b0 < D.nmaxb; b0 += 4 * blockDim.x) {
     unsigned long long qv[4], qn[4];
@@ -926,23 +1117,16 @@ __global__ void __launch_bounds__(256) k_timestep(Dev D) {
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int b = b0 + u * (int)blockDim.x;
       mv = (qv[u] > mv) ? qv[u] : mv;
       mn = (qn[u] > mn) ? qn[u] : mn;
-      if (b < D.nmaxb) {
-        D.maxb[b] = 0ull;
-        D.maxb[D.nmaxb + b] = 0ull;
-      }
     }
   }
   block_max2_u64(mv, mn);
   if (threadIdx.x != 0) return;
   const DevParams& P = D.P;
   DevScalars* S = D.sc;
-  if (mv > S->umax_bits) S->umax_bits = mv;
-  if (mn > S->numax_bits) S->numax_bits = mn;
-  S->umax = __longlong_as_double((long long)S->umax_bits);
-  S->numax = __longlong_as_double((long long)S->numax_bits);
+  S->umax = __longlong_as_double((long long)mv);
+  S->numax = __longlong_as_double((long long)mn);
   const double dl = P.dl, sig1 = P.sig1;
   if (!P.fixed_dt) {
     const double t = S->t;
@@ -976,13 +1160,15 @@ __global__ void __launch_bounds__(256) k_timestep(Dev D) {
   S->dtsol = (dtsol < S->dt) ? dtsol : S->dt;
   S->solsteps = solsteps;
 }
+__global__ void __launch_bounds__(256) k_timestep(Dev D) {
+  if (blockIdx.x == 0) timestep_block(D);
+}
 
 // ---------------------------------------------------------------------
 // scalart_anisotropic / scalarupdate, scalar.f90:242-397.  The neighbour
 // list (and its r) is from the last neighbour search; positions are current.
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_scalart(Dev D) {
-  PIDX();
+__device__ __forceinline__ void scalart_body(const Dev& D, const int i, const int n) {
   if (i >= n || D.tid[i] != TID_FLUID) return;
   const DevParams& P = D.P;
   const double al = 0.0025, at = 0.00025, Dm = (double)1e-9f, thr = 0.7;
@@ -1047,12 +1233,19 @@ __global__ void __launch_bounds__(256) k_scalart(Dev D) {
   }
   D.cdiff[i] = cd;
 }
+__global__ void __launch_bounds__(256) k_scalart(Dev D) {
+  PIDX();
+  scalart_body(D, i, n);
+}
 
 // stime < 0: use the sub-step length dtsol from the last timestep
-__global__ void k_scalarupdate(Dev D, double stime) {
-  PIDX();
+__device__ __forceinline__ void scalarupdate_body(const Dev& D, const int i, const int n, double stime) {
   if (i >= n || D.tid[i] != TID_FLUID) return;
   const double st = (stime < 0.0) ? D.sc->dtsol : stime;
   D.c[i] = D.c[i] + D.cdiff[i] * st;
   if ((D.flags[i] & F_BUFFER) && (D.x[i] < 1.6)) D.c[i] = 0.5;
+}
+__global__ void k_scalarupdate(Dev D, double stime) {
+  PIDX();
+  scalarupdate_body(D, i, n, stime);
 }

@@ -17,6 +17,7 @@
 #include "../../include/isph_abi.h"
 #include "isph_device.h"
 #include "isph_kernels.h"
+#include "isph_phases.h"
 #include "isph_solver.h"
 #include "isph_mg.h"
 #include "isph_batch.h"
@@ -28,10 +29,27 @@ static_assert(NBR_CAP == ISPH_NBR_CAP, "neighbour capacity mismatch");
 // HIP-event timing of one kernel class
 struct KTimer {
   const char* name;
+  int per_iter;  // launches per solver iteration (the working launches of a chunk)
   double total_ms = 0;
   int64_t launches = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pool, pending;
 };
+
+// Kernels whose block size the autotuner chooses, in the order of
+// isph_autotune's output (their results do not depend on it: one thread per
+// particle, and only integer or max reductions across a block).
+enum {
+  TK_NEIGHBOUR, TK_EFFPOR, TK_EDDYVIS, TK_VARTS, TK_INTVEL, TK_ASSEMBLE, TK_COMPVEL,
+  TK_SHIFT, TK_SCALART, TK_N
+};
+static_assert(TK_N == ISPH_TUNE_N, "tuned kernel count mismatch");
+// launch shapes the tuner chooses from: particles in blocks of 64, 128 or
+// 256 threads; for the neighbour search also cell tiles of 4 rows on 128
+// threads or 8 rows on 256 (LDS-staged)
+enum { TS_P64, TS_P128, TS_P256, TS_TILE4, TS_TILE8, TS_N };
+static_assert(TS_N == ISPH_TUNE_C, "launch shape count mismatch");
+static const int ts_block[TS_N] = {64, 128, 256, 128, 256};
+static int ts_count(int tk) { return tk == TK_NEIGHBOUR ? TS_N : TS_TILE4; }
 
 // an inflow/outflow buffer zone: device copies of the template and its cell
 // tables, and the insertion clock (main.f90:217-235)
@@ -53,7 +71,8 @@ struct isph_ctx {
   bool wave = false;
   double wv_stroke = 0.0, wv_period = 1.0, wv_ramp = 0.0, wv_t0 = 0.0;
   bool prof = false;
-  KTimer timers[4] = {{"spmv_p"}, {"spmv_s"}, {"spmv_p32"}, {"spmv_s32"}};
+  KTimer timers[7] = {{"spmv_p", 1},    {"spmv_s", 1},    {"spmv_p32", 1}, {"spmv_s32", 1},
+                      {"mg_coarse", 2}, {"mg_pre0", 2},   {"mg_post0", 2}};
   Dev D;
   hipStream_t stream = nullptr;
   int64_t cap = 0, scap = 0;
@@ -71,6 +90,9 @@ struct isph_ctx {
   // previous precision solve: iterations to the first stop, and after it
   // (chunk sizing only; the results do not depend on the chunks)
   int pred_first = 0, pred_after = 0;
+  bool in_batch = false;   // a member of a batch: its form (precision, WCSPH) is fixed
+  int tshape[TK_N] = {TS_P256, TS_P256, TS_P256, TS_P256, TS_P256,
+                     TS_P256, TS_P256, TS_P256, TS_P256};  // tuned launch shapes
 };
 
 #define HIPCHK(ctx, expr)                                                     \
@@ -170,7 +192,8 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   // vectors, level-0 multigrid vectors, setup rows
   if (precision) per += 2 * 4 * ((ROW_CAP + 1) / 2) + 5 * 4 + 2 * 4 + 9 * 8;
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
-  b += 2 * 8 * 2 * (n_cap / 256 + 2);           // reduction partials, running maxima
+  b += 2 * 8 * (n_cap / 256 + 2);               // reduction partials
+  b += 8 * (2 * ncell + 2 * (ncell / 256 + 2)); // per-cell maxima and their block maxima
   // cell levels: stencil, x, b, r and 1/diagonal per unknown, the levels
   // holding 4/3 of the cell count
   if (precision) b += 4 * 13 * ((4 * ncell) / 3 + MG_MAX_LEVELS) + 8 * MG_DENSE_MAX * MG_DENSE_MAX;
@@ -236,10 +259,18 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   rc |= dalloc(c, &D.srho, c->scap);
   rc |= dalloc(c, &D.sstart, c->ncell + 1);
   rc |= dalloc(c, &D.part, 2 * (N / 256 + 2));
-  D.nmaxb = (int)(N / 256 + 2);
+  D.nmaxb = (int)(c->ncell / 256 + 2);
   rc |= dalloc(c, &D.maxb, 2 * (int64_t)D.nmaxb);
-  if (rc == 0 && hipMemset(D.maxb, 0, sizeof(unsigned long long) * 2 * (size_t)D.nmaxb) != hipSuccess)
-    rc |= ISPH_EHIP;  // the running maxima start at zero
+  rc |= dalloc(c, &D.cmaxv, c->ncell);
+  rc |= dalloc(c, &D.cmaxn, c->ncell);
+  // the per-cell maxima start at zero (an empty cell keeps them until it fills)
+  if (rc == 0 && (hipMemset(D.maxb, 0, sizeof(unsigned long long) * 2 * (size_t)D.nmaxb) !=
+                      hipSuccess ||
+                  hipMemset(D.cmaxv, 0, sizeof(unsigned long long) * (size_t)c->ncell) !=
+                      hipSuccess ||
+                  hipMemset(D.cmaxn, 0, sizeof(unsigned long long) * (size_t)c->ncell) !=
+                      hipSuccess))
+    rc |= ISPH_EHIP;
   rc |= dalloc(c, &D.sc, 1);
   rc |= dalloc(c, &c->d_bn2, 1);
   if (params->precision) {
@@ -346,6 +377,10 @@ extern "C" const char* isph_last_error(const isph_ctx* c) {
 extern "C" int isph_set_params(isph_ctx* c, const isph_params* p) {
   if (!c || !p) return ISPH_EARG;
   if (p->cellx != c->P.cellx || p->celly != c->P.celly) return ISPH_EARG;
+  if (c->in_batch && p->precision != c->P.precision) {
+    set_err(c, "the solve form of a context cannot change while it belongs to a batch");
+    return ISPH_ESTATE;
+  }
   if (p->precision && !c->D.av16) {
     set_err(c, "context was created without the precision-form buffers");
     return ISPH_EARG;
@@ -402,6 +437,8 @@ extern "C" int isph_upload(isph_ctx* c, int64_t n, const double* x, const double
   }
   if (n > 0 && (!x || !y || !u || !v || !p || !m || !rho || !rho0 || !vol0 || !cc || !tid || !flags))
     return ISPH_EARG;
+  // kernels still in flight on the context's stream read the arrays written here
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   // initial binning: walls before fluid in each cell, input order kept
   std::vector<int64_t> key(n);
   for (int64_t i = 0; i < n; i++) {
@@ -480,6 +517,7 @@ extern "C" int isph_upload_solids(isph_ctx* c, int64_t ns, const double* x, cons
     set_err(c, "upload of %lld solids exceeds capacity %lld", (long long)ns, (long long)c->scap);
     return ISPH_ECAP;
   }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   std::vector<int32_t> key(ns);
   std::vector<int32_t> cs(c->ncell + 1, 0);
   for (int64_t i = 0; i < ns; i++) {
@@ -584,6 +622,19 @@ extern "C" int isph_set_scalars(isph_ctx* c, const isph_scalars* in) {
     }                                                                                \
   } while (0)
 
+// launch of a tuned particle kernel with the context's block size for it
+#define TLAUNCH(tk, kern, nthreads, ...)                                             \
+  do {                                                                               \
+    const int b_ = ts_block[c->tshape[tk]];                                          \
+    hipLaunchKernelGGL(kern, dim3(std::max(1, ((nthreads) + b_ - 1) / b_)), dim3(b_), 0, \
+                       c->stream, __VA_ARGS__);                                      \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess) {                                                          \
+      set_err(c, "launch of %s failed: %s", #kern, hipGetErrorString(e_));          \
+      return ISPH_EHIP;                                                              \
+    }                                                                                \
+  } while (0)
+
 static std::pair<hipEvent_t, hipEvent_t> timer_pair(KTimer& t) {
   if (!t.pool.empty()) {
     auto p = t.pool.back();
@@ -597,16 +648,18 @@ static std::pair<hipEvent_t, hipEvent_t> timer_pair(KTimer& t) {
 }
 
 // Collect the elapsed times of completed launches (called after a sync).
-// Only the first `keep` pending launches of each timer are counted: the
-// solver launches whole chunks of iterations, and the launches after the
-// device has set its done flag return at once; they are dropped so that the
-// average is over launches that did the full work.  keep < 0 counts all.
+// Only the launches of the first `keep` pending iterations of each timer are
+// counted (per_iter launches each): the solver launches whole chunks of
+// iterations, and the launches after the device has set its done flag
+// return at once; they are dropped so that the average is over launches
+// that did the full work.  keep < 0 counts all.
 static void timers_collect(isph_ctx* c, int64_t keep) {
   for (KTimer& t : c->timers) {
     int64_t k = 0;
     for (auto& pr : t.pending) {
       float ms = 0.f;
-      if ((keep < 0 || k < keep) && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+      if ((keep < 0 || k < keep * t.per_iter) &&
+          hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
         t.total_ms += ms;
         t.launches += 1;
       }
@@ -702,13 +755,7 @@ extern "C" int isph_projection(isph_ctx* c) {
   return 0;
 }
 
-static void swap_state(Dev& D) {
-  std::swap(D.x, D.x2); std::swap(D.y, D.y2); std::swap(D.u, D.u2); std::swap(D.v, D.v2);
-  std::swap(D.us, D.us2); std::swap(D.vs, D.vs2); std::swap(D.xs, D.xs2); std::swap(D.ys, D.ys2);
-  std::swap(D.p, D.p2); std::swap(D.m, D.m2); std::swap(D.rho, D.rho2); std::swap(D.rho0, D.rho02);
-  std::swap(D.vol0, D.vol02); std::swap(D.c, D.c2); std::swap(D.tid, D.tid2);
-  std::swap(D.flags, D.flags2); std::swap(D.vic, D.vic2); std::swap(D.cell, D.cell2);
-}
+static void swap_state(Dev& D) { dev_swap_state(D); }
 
 static int scan_bins(isph_ctx* c, int len) {
   const int ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
@@ -744,8 +791,30 @@ extern "C" int isph_resetid(isph_ctx* c) {
   return c ? 0 : ISPH_EARG;
 }
 
+// the neighbour search in the context's tuned launch shape
+static int launch_neighbour(isph_ctx* c) {
+  const int sh = c->tshape[TK_NEIGHBOUR];
+  if (sh == TS_TILE4 || sh == TS_TILE8) {
+    const int rows = sh == TS_TILE4 ? 4 : 8;
+    const dim3 grid(std::max(1, ntile_count(rows, c->D.P.cellx, c->D.P.celly)));
+    if (sh == TS_TILE4)
+      hipLaunchKernelGGL(k_neighbour_tile4, grid, dim3(128), 0, c->stream, c->D);
+    else
+      hipLaunchKernelGGL(k_neighbour_tile8, grid, dim3(256), 0, c->stream, c->D);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_err(c, "launch of the tiled neighbour search failed: %s", hipGetErrorString(e));
+      return ISPH_EHIP;
+    }
+    return 0;
+  }
+  TLAUNCH(TK_NEIGHBOUR, k_neighbour, c->n_hi, c->D);
+  return 0;
+}
+
 static int neighbour_nocheck(isph_ctx* c) {
-  LAUNCH(k_neighbour, c->n_hi, c->D);
+  const int rc = launch_neighbour(c);
+  if (rc) return rc;
   c->have_nbr = true;
   return 0;
 }
@@ -759,20 +828,20 @@ extern "C" int isph_neighbour(isph_ctx* c) {
 
 extern "C" int isph_effpor(isph_ctx* c) {
   if (!c) return ISPH_EARG;
-  LAUNCH(k_effpor, c->n_hi, c->D);
+  TLAUNCH(TK_EFFPOR, k_effpor, c->n_hi, c->D);
   return 0;
 }
 extern "C" int isph_eddyvis(isph_ctx* c) {
   if (!c) return ISPH_EARG;
   if (!c->have_nbr) { set_err(c, "eddyvis before neighbour"); return ISPH_ESTATE; }
-  LAUNCH(k_eddyvis, c->n_hi, c->D);
+  TLAUNCH(TK_EDDYVIS, k_eddyvis, c->n_hi, c->D);
   return 0;
 }
 extern "C" int isph_int_vel(isph_ctx* c) {
   if (!c) return ISPH_EARG;
   if (!c->have_nbr) { set_err(c, "int_vel before neighbour"); return ISPH_ESTATE; }
-  LAUNCH(k_int_vel_varts, c->n_hi, c->D);
-  LAUNCH(k_int_vel, c->n_hi, c->D);
+  TLAUNCH(TK_VARTS, k_int_vel_varts, c->n_hi, c->D);
+  TLAUNCH(TK_INTVEL, k_int_vel, c->n_hi, c->D);
   return 0;
 }
 
@@ -782,7 +851,7 @@ __global__ void k_zero_nnz(Dev D) {
 
 static int assemble_nocheck(isph_ctx* c) {
   hipLaunchKernelGGL(k_zero_nnz, dim3(1), dim3(64), 0, c->stream, c->D);
-  LAUNCH(k_ppe_assemble, c->n_hi, c->D);
+  TLAUNCH(TK_ASSEMBLE, k_ppe_assemble, c->n_hi, c->D);
   c->assembled = true;
   return 0;
 }
@@ -906,12 +975,12 @@ static int bh_gather(BatchHost* b) {
     }                                                                                 \
   } while (0)
 
-// A timed batched row launch (events attached to the dispatch, as TIMED_LAUNCH)
-#define TIMED_BLAUNCH(tidx, kern, grid, ...)                                          \
+// A timed batched launch (events attached to the dispatch, as TIMED_LAUNCH)
+#define TIMED_BLAUNCH_LDS(tidx, kern, grid, block, lds, ...)                          \
   do {                                                                                \
     if (c->prof) {                                                                    \
       std::pair<hipEvent_t, hipEvent_t> ev_ = timer_pair(c->timers[tidx]);            \
-      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, ev_.first,     \
+      hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, c->stream, ev_.first, \
                             ev_.second, 0, __VA_ARGS__);                              \
       hipError_t e_ = hipGetLastError();                                              \
       if (e_ != hipSuccess) {                                                         \
@@ -920,9 +989,11 @@ static int bh_gather(BatchHost* b) {
       }                                                                               \
       c->timers[tidx].pending.push_back(ev_);                                         \
     } else {                                                                          \
-      BLAUNCH(kern, grid, 256, __VA_ARGS__);                                          \
+      BLAUNCH_LDS(kern, grid, block, lds, __VA_ARGS__);                               \
     }                                                                                 \
   } while (0)
+#define TIMED_BLAUNCH(tidx, kern, grid, ...) \
+  TIMED_BLAUNCH_LDS(tidx, kern, grid, 256, 0, __VA_ARGS__)
 
 // one V-cycle for every scenario: WHICH 0 gives r32 = M(pp32), 1 sv32 = M(sv);
 // OFF: level-0 rows read the packed 16-bit column offsets
@@ -932,14 +1003,14 @@ static int bh_vcycle(BatchHost* b, const BatchDev& B, int grow) {
   isph_ctx* c = b->ctx[0];
   const int S = b->S;
   auto tot = [&](int kind) { return b->pre[(size_t)kind * (S + 1) + S]; };
-  BLAUNCH(kb_mg_pre0<V>, grow, 256, B);
+  TIMED_BLAUNCH(5, kb_mg_pre0<V>, grow, B);
   BLAUNCH(kb_mg_restrict0, tot(BK_CELL1), 256, B);
   for (int l = 1; l <= b->max_nmb; l++)
     if (tot(BK_DOWN + l - 1) > 0) BLAUNCH(kb_mg_down, tot(BK_DOWN + l - 1), 256, B, l);
-  BLAUNCH_LDS(kb_mg_coarse, S, MG_CO_THREADS, b->co_lds, B);
+  TIMED_BLAUNCH_LDS(4, kb_mg_coarse, S, MG_CO_THREADS, b->co_lds, B);
   for (int l = b->max_nmb; l >= 1; l--)
     if (tot(BK_UP + l - 1) > 0) BLAUNCH(kb_mg_up, tot(BK_UP + l - 1), 256, B, l);
-  BLAUNCH(kb_mg_post0<V>, grow, 256, B);
+  TIMED_BLAUNCH(6, kb_mg_post0<V>, grow, B);
   return 0;
 }
 
@@ -995,6 +1066,14 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 1);
   rc = bh_gather(b);
   if (rc) return rc;
+  // error bits of the phases before the solve (neighbour capacity, PPE
+  // diagonal) end the step, as the per-scenario check does
+  for (int s = 0; s < S; s++) {
+    if (b->sc[s].err) {
+      rc = check_err_flags(b->ctx[s]);
+      if (rc) return rc;
+    }
+  }
   // packed 16-bit column offsets unless some scenario's rows need int32
   bool off16 = true;
   for (int s = 0; s < S; s++) off16 = off16 && !b->sc[s].far;
@@ -1212,7 +1291,8 @@ extern "C" int isph_ppesolve(isph_ctx* c, isph_solve_stats* stats) {
 
 extern "C" int isph_comp_vel(isph_ctx* c) {
   if (!c) return ISPH_EARG;
-  LAUNCH(k_comp_vel, c->n_hi, c->D);
+  TLAUNCH(TK_COMPVEL, k_comp_vel, c->n_hi, c->D);
+  LAUNCH(k_cell_max, c->ncell, c->D, c->ncell, 0);
   return 0;
 }
 extern "C" int isph_comp_pos(isph_ctx* c) {
@@ -1227,7 +1307,7 @@ extern "C" int isph_freesurf(isph_ctx* c) {
 }
 extern "C" int isph_opt2_shift(isph_ctx* c) {
   if (!c) return ISPH_EARG;
-  LAUNCH(k_opt2_shift, c->n_hi, c->D);
+  TLAUNCH(TK_SHIFT, k_opt2_shift, c->n_hi, c->D);
   LAUNCH(k_shift_apply, c->n_hi, c->D);
   return 0;
 }
@@ -1237,29 +1317,16 @@ extern "C" int isph_densityupdate(isph_ctx* c) {
   return 0;
 }
 
-// clears the running maxima of timestep; with `slots`, also the row blocks'
-// pending maxima (timestep's fold clears those itself)
-__global__ void k_reset_max(Dev D, int slots) {
-  if (blockIdx.x != 0) return;
-  if (threadIdx.x == 0) {
-    D.sc->umax_bits = 0ull;
-    D.sc->numax_bits = 0ull;
-  }
-  if (slots)
-    for (int b = threadIdx.x; b < 2 * D.nmaxb; b += blockDim.x) D.maxb[b] = 0ull;
-}
-
 extern "C" int isph_timestep(isph_ctx* c) {
   if (!c) return ISPH_EARG;
   hipLaunchKernelGGL(k_timestep, dim3(1), dim3(256), 0, c->stream, c->D);
-  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(64), 0, c->stream, c->D, 0);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
 extern "C" int isph_scalart_anisotropic(isph_ctx* c) {
   if (!c) return ISPH_EARG;
   if (!c->have_nbr) { set_err(c, "scalar transport before neighbour"); return ISPH_ESTATE; }
-  LAUNCH(k_scalart, c->n_hi, c->D);
+  TLAUNCH(TK_SCALART, k_scalart, c->n_hi, c->D);
   return 0;
 }
 
@@ -1269,20 +1336,13 @@ extern "C" int isph_scalarupdate(isph_ctx* c, double stime) {
   return 0;
 }
 
-__global__ void __launch_bounds__(256) k_numax_init(Dev D) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) D.sc->umax_bits = 0ull;
-  block_running_max(D, 0ull, (i < D.sc->n) ? max_bits(D.nut[i]) : 0ull);
-}
-
 extern "C" int isph_init_state(isph_ctx* c) {
   if (!c) return ISPH_EARG;
   int rc = isph_neighbour(c);
   if (rc) return rc;
-  LAUNCH(k_effpor, c->n_hi, c->D);
-  LAUNCH(k_eddyvis, c->n_hi, c->D);
-  hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(256), 0, c->stream, c->D, 1);
-  LAUNCH(k_numax_init, c->n_hi, c->D);
+  TLAUNCH(TK_EFFPOR, k_effpor, c->n_hi, c->D);
+  TLAUNCH(TK_EDDYVIS, k_eddyvis, c->n_hi, c->D);
+  LAUNCH(k_cell_max, c->ncell, c->D, c->ncell, 1);
   rc = isph_timestep(c);
   if (rc) return rc;
   return sync(c);
@@ -1302,6 +1362,7 @@ extern "C" int isph_set_buffer(isph_ctx* c, int32_t zone, int64_t nt, const doub
   if (!c || zone < 1 || zone > 2 || nt < 0 || nt > c->cap) return ISPH_EARG;
   if (nt > 0 && (!x || !y || !u || !v || !p)) return ISPH_EARG;
   HostZone& Z = c->zones[zone - 1];
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // the zone's arrays may be in use
   std::vector<int32_t> tcell(nt), trank(nt), tcnt(c->ncell, 0);
   std::vector<uint8_t> entry(c->ncell, 0);
   for (int64_t t = 0; t < nt; t++) {
@@ -1388,6 +1449,10 @@ extern "C" int isph_buffer_update(isph_ctx* c, int32_t zone) {
 extern "C" int isph_set_wcsph(isph_ctx* c, double c0, double gamma, double delta,
                               double swl, int32_t in_step) {
   if (!c || !(c0 > 0.0) || !(gamma > 0.0)) return ISPH_EARG;
+  if (c->in_batch) {
+    set_err(c, "the weakly compressible form cannot be set while the context belongs to a batch");
+    return ISPH_ESTATE;
+  }
   c->D.P.c0 = c0;
   c->D.P.gam = gamma;
   c->D.P.delt = delta;
@@ -1514,7 +1579,7 @@ static int step_pre(isph_ctx* c) {
   }
   hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, c->stream, c->D);
   for (int q = 0; q < s.solsteps; q++) {
-    LAUNCH(k_scalart, c->n_hi, c->D);
+    TLAUNCH(TK_SCALART, k_scalart, c->n_hi, c->D);
     LAUNCH(k_scalarupdate, c->n_hi, c->D, -1.0);
   }
   rc = buffer_clock(c, s.t, s.t + s.dt);  // the device's t + dt, the same double sum
@@ -1528,10 +1593,10 @@ static int step_pre(isph_ctx* c) {
   if (rc) return rc;
   rc = neighbour_nocheck(c);
   if (rc) return rc;
-  LAUNCH(k_effpor, c->n_hi, c->D);
-  LAUNCH(k_eddyvis, c->n_hi, c->D);
-  LAUNCH(k_int_vel_varts, c->n_hi, c->D);
-  LAUNCH(k_int_vel, c->n_hi, c->D);
+  TLAUNCH(TK_EFFPOR, k_effpor, c->n_hi, c->D);
+  TLAUNCH(TK_EDDYVIS, k_eddyvis, c->n_hi, c->D);
+  TLAUNCH(TK_VARTS, k_int_vel_varts, c->n_hi, c->D);
+  TLAUNCH(TK_INTVEL, k_int_vel, c->n_hi, c->D);
   if (c->wcsph) {
     // continuity and the Tait law in place of the pressure Poisson solve
     LAUNCH(k_calmasscons, c->n_hi, c->D);
@@ -1545,20 +1610,21 @@ static int step_pre(isph_ctx* c) {
 }
 
 static int step_post(isph_ctx* c) {
-  LAUNCH(k_comp_vel, c->n_hi, c->D);
+  TLAUNCH(TK_COMPVEL, k_comp_vel, c->n_hi, c->D);
+  LAUNCH(k_cell_max, c->ncell, c->D, c->ncell, 0);
   LAUNCH(k_comp_pos, c->n_hi, c->D);
   int rc = rebin(c, 1);
   if (rc) return rc;
   rc = neighbour_nocheck(c);
   if (rc) return rc;
   LAUNCH(k_freesurf, c->n_hi, c->D);
-  LAUNCH(k_effpor, c->n_hi, c->D);
-  LAUNCH(k_opt2_shift, c->n_hi, c->D);
+  TLAUNCH(TK_EFFPOR, k_effpor, c->n_hi, c->D);
+  TLAUNCH(TK_SHIFT, k_opt2_shift, c->n_hi, c->D);
   LAUNCH(k_shift_apply, c->n_hi, c->D);
   if (!c->wcsph) LAUNCH(k_densityupdate, c->n_hi, c->D);
   rc = isph_timestep(c);
   if (rc) return rc;
-  LAUNCH(k_eddyvis, c->n_hi, c->D);
+  TLAUNCH(TK_EDDYVIS, k_eddyvis, c->n_hi, c->D);
   return check_err_flags(c);
 }
 
@@ -1694,6 +1760,7 @@ extern "C" int isph_ppe_get_solution(isph_ctx* c, double* x) {
 extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, const int32_t* col,
                                 const double* val, const double* rhs, const double* x0) {
   if (!c || n < 0 || n > c->cap) return ISPH_EARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   std::vector<int32_t> L(n), C((size_t)ROW_CAP * c->cap, 0);
   std::vector<double> V((size_t)ROW_CAP * c->cap, 0.0);
   for (int i = 0; i < n; i++) {
@@ -1724,12 +1791,35 @@ extern "C" int isph_load_system(isph_ctx* c, int32_t n, const int32_t* len, cons
 // ---------------------------------------------------------------------
 // batched form: independent scenarios advanced together
 // ---------------------------------------------------------------------
-// Batched form: independent scenarios stepped together.  Every context of a
-// batch is switched to one shared stream; the phases before and after the
-// PPE solve are launched per scenario, and the precision-form solve of all
-// scenarios runs as one set of launches (bh_solve).  Contexts in the FP64
-// form are solved one after the other; weakly compressible contexts (set
-// before the batch is made) have no solve.
+// Every context of a batch is switched to one shared stream.  When every
+// scenario is an ISPH scenario without an insertion clock on its buffer
+// zones, a batched step runs each particle phase as one launch over all
+// scenarios (isph_phases.h) and the precision-form solves as one set of
+// launches (bh_solve): the host reads every scenario's scalars once at the
+// start of a step (solsteps, the paddle's time, error bits) and otherwise
+// only polls the solver.  Otherwise the phases run per scenario.  Contexts in
+// the FP64 form are solved one after the other.
+struct PhaseHost {
+  int S = 0;
+  Dev* d_devs = nullptr;          // scenario tables (swapped in place on the device)
+  int* d_pre = nullptr;           // PH_NKIND x (S+1) block prefixes
+  double* d_wave = nullptr;       // 3S paddle values of the step
+  DevScalars* d_sc = nullptr;     // gathered scalars
+  Dev* h_devs = nullptr;          // pinned host staging of the same
+  int* h_pre = nullptr;
+  double* h_wave = nullptr;
+  DevScalars* h_sc = nullptr;
+  int64_t launches = 0;           // launches of the last batched step
+  // HIP graphs of the fixed-shape phase sequences (before and after the
+  // solve), replayed while the launch shapes stay the same
+  bool use_graph = false;
+  hipGraphExec_t gpre = nullptr, gpost = nullptr;
+  int gkey[PH_NKIND] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  int64_t gcount[2] = {0, 0};     // kernels in each graph
+  int tshape[TK_N] = {TS_P256, TS_P256, TS_P256, TS_P256, TS_P256,
+                     TS_P256, TS_P256, TS_P256, TS_P256};  // tuned launch shapes
+};
+
 struct isph_batch {
   std::vector<isph_ctx*> ctx;
   std::vector<hipStream_t> own;  // each context's stream, restored on destroy
@@ -1737,7 +1827,374 @@ struct isph_batch {
   BatchHost* pb = nullptr;       // the precision-form contexts as one solve batch
   std::vector<isph_ctx*> pctx;
   std::vector<isph_solve_stats> last;
+  PhaseHost* ph = nullptr;       // batched particle phases (null: per-scenario phases)
 };
+
+static void ph_drop_graphs(PhaseHost* h) {
+  if (h->gpre) (void)hipGraphExecDestroy(h->gpre);
+  if (h->gpost) (void)hipGraphExecDestroy(h->gpost);
+  h->gpre = h->gpost = nullptr;
+  for (int& k : h->gkey) k = -1;
+}
+
+static void ph_free(PhaseHost* h) {
+  if (!h) return;
+  ph_drop_graphs(h);
+  (void)hipFree(h->d_devs);
+  (void)hipFree(h->d_pre);
+  (void)hipFree(h->d_wave);
+  (void)hipFree(h->d_sc);
+  (void)hipHostFree(h->h_devs);
+  (void)hipHostFree(h->h_pre);
+  (void)hipHostFree(h->h_wave);
+  (void)hipHostFree(h->h_sc);
+  delete h;
+}
+
+static int ph_make(int S, PhaseHost** out) {
+  PhaseHost* h = new PhaseHost();
+  h->S = S;
+  const size_t npre = (size_t)PH_NKIND * (S + 1);
+  hipError_t e = hipMalloc((void**)&h->d_devs, sizeof(Dev) * S);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_pre, sizeof(int) * npre);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_wave, sizeof(double) * 3 * S);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_sc, sizeof(DevScalars) * S);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_devs, sizeof(Dev) * S);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_pre, sizeof(int) * npre);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_wave, sizeof(double) * 3 * S);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_sc, sizeof(DevScalars) * S);
+  if (e != hipSuccess) {
+    ph_free(h);
+    return ISPH_EHIP;
+  }
+  *out = h;
+  return 0;
+}
+
+#define PLAUNCH(kern, grid, ...)                                                       \
+  do {                                                                                 \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c0->stream, __VA_ARGS__);       \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess) {                                                            \
+      set_err(c0, "launch of %s failed: %s", #kern, hipGetErrorString(e_));           \
+      return ISPH_EHIP;                                                                \
+    }                                                                                  \
+    h->launches++;                                                                     \
+  } while (0)
+
+// the scenario tables as the host holds them (every kernel of the batched
+// phases reaches a scenario's arrays and scalars through them, so they are
+// written before any of them runs: at batch creation and at every step)
+static int ph_upload_devs(isph_batch* b) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  for (int s = 0; s < h->S; s++) h->h_devs[s] = b->ctx[s]->D;
+  HIPCHK(c0, hipMemcpyAsync(h->d_devs, h->h_devs, sizeof(Dev) * h->S, hipMemcpyHostToDevice,
+                            c0->stream));
+  return 0;
+}
+
+// launch of a tuned particle phase with the batch's block size for it
+#define TPLAUNCH(tk, kern, ...)                                                        \
+  do {                                                                                 \
+    const int b_ = ts_block[h->tshape[tk]];                                            \
+    const int kind_ = b_ == 64 ? PH_P64 : (b_ == 128 ? PH_P128 : PH_PART);             \
+    hipLaunchKernelGGL(kern, dim3(h->h_pre[kind_ * (h->S + 1) + h->S]), dim3(b_), 0,   \
+                       c0->stream, __VA_ARGS__);                                       \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess) {                                                            \
+      set_err(c0, "launch of %s failed: %s", #kern, hipGetErrorString(e_));           \
+      return ISPH_EHIP;                                                                \
+    }                                                                                  \
+    h->launches++;                                                                     \
+  } while (0)
+
+// the batched neighbour search in the batch's tuned launch shape
+static int ph_neighbour(isph_batch* b, const PhaseBatch& B) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const int sh = h->tshape[TK_NEIGHBOUR];
+  if (sh == TS_TILE4 || sh == TS_TILE8) {
+    const int kind = sh == TS_TILE4 ? PH_T4 : PH_T8;
+    const dim3 grid(std::max(1, h->h_pre[kind * (h->S + 1) + h->S]));
+    if (sh == TS_TILE4)
+      hipLaunchKernelGGL(kb_neighbour_tile4, grid, dim3(128), 0, c0->stream, B);
+    else
+      hipLaunchKernelGGL(kb_neighbour_tile8, grid, dim3(256), 0, c0->stream, B);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_err(c0, "launch of the tiled neighbour search failed: %s", hipGetErrorString(e));
+      return ISPH_EHIP;
+    }
+    h->launches++;
+    return 0;
+  }
+  TPLAUNCH(TK_NEIGHBOUR, kb_neighbour, B);
+  return 0;
+}
+
+// one read of every scenario's scalars (the device tables are current)
+static int ph_gather(isph_batch* b) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const PhaseBatch B{h->S, h->d_devs, h->d_pre, h->d_wave};
+  PLAUNCH(kb_gather_phase, (h->S + 63) / 64, B, h->d_sc);
+  HIPCHK(c0, hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(DevScalars) * h->S, hipMemcpyDeviceToHost,
+                            c0->stream));
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));
+  return 0;
+}
+
+// the step's tables: each scenario's Dev with its row blocks for the live
+// count just read, the block prefixes of its index spaces, its paddle values
+static int ph_upload(isph_batch* b) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const int S = h->S;
+  for (int k = 0; k < PH_NKIND; k++) h->h_pre[k * (S + 1)] = 0;
+  for (int s = 0; s < S; s++) {
+    const isph_ctx* c = b->ctx[s];
+    h->h_devs[s] = c->D;
+    const int nb = c->ncell * NCLS + 1;
+    const int blocks[PH_NKIND] = {c->D.nblk, grid_of(c->ncell + 1), grid_of(nb),
+                                  (nb + SCAN_TILE - 1) / SCAN_TILE,
+                                  std::max(1, (c->n_hi + 63) / 64),
+                                  std::max(1, (c->n_hi + 127) / 128),
+                                  ntile_count(4, c->D.P.cellx, c->D.P.celly),
+                                  ntile_count(8, c->D.P.cellx, c->D.P.celly)};
+    for (int k = 0; k < PH_NKIND; k++)
+      h->h_pre[k * (S + 1) + s + 1] = h->h_pre[k * (S + 1) + s] + blocks[k];
+  }
+  HIPCHK(c0, hipMemcpyAsync(h->d_devs, h->h_devs, sizeof(Dev) * S, hipMemcpyHostToDevice,
+                            c0->stream));
+  HIPCHK(c0, hipMemcpyAsync(h->d_pre, h->h_pre, sizeof(int) * PH_NKIND * (S + 1),
+                            hipMemcpyHostToDevice, c0->stream));
+  HIPCHK(c0, hipMemcpyAsync(h->d_wave, h->h_wave, sizeof(double) * 3 * S,
+                            hipMemcpyHostToDevice, c0->stream));
+  return 0;
+}
+
+// rebin of every scenario (the counting sort of rebin(), batched)
+static int ph_rebin(isph_batch* b, const PhaseBatch& B, int alt) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const int S = h->S;
+  auto tot = [&](int kind) { return h->h_pre[kind * (S + 1) + S]; };
+  PLAUNCH(kb_fill_bins, tot(PH_BIN), B);
+  PLAUNCH(kb_rebin_classify, tot(PH_PART), B, alt);
+  PLAUNCH(kb_rebin_rank, tot(PH_PART), B);
+  PLAUNCH(kb_scan_tiles, tot(PH_TILE), B);
+  PLAUNCH(kb_scan_sums, S, B);
+  PLAUNCH(kb_scan_add, tot(PH_BIN), B);
+  PLAUNCH(kb_rebin_scatter, tot(PH_PART), B);
+  PLAUNCH(kb_swap, (S + 63) / 64, B);
+  PLAUNCH(kb_rebin_finish, tot(PH_CELL), B);
+  return 0;
+}
+
+// the host's half of a batched rebin: the ping-pong swap the device did
+static void ph_rebin_host(isph_batch* b) {
+  for (isph_ctx* c : b->ctx) {
+    swap_state(c->D);
+    c->assembled = false;
+    c->have_nbr = false;
+  }
+}
+
+// the phases from projection to the PPE assembly (after the scalar sub-steps)
+static int ph_pre_launch(isph_batch* b, const PhaseBatch& B) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const int S = h->S;
+  auto tot = [&](int kind) { return h->h_pre[kind * (S + 1) + S]; };
+  PLAUNCH(kb_projection, tot(PH_PART), B);
+  PLAUNCH(kb_paddle, tot(PH_PART), B);
+  int rc = ph_rebin(b, B, 0);
+  if (rc == 0) rc = ph_neighbour(b, B);
+  if (rc) return rc;
+  TPLAUNCH(TK_EFFPOR, kb_effpor, B);
+  TPLAUNCH(TK_EDDYVIS, kb_eddyvis, B);
+  TPLAUNCH(TK_VARTS, kb_int_vel_varts, B);
+  TPLAUNCH(TK_INTVEL, kb_int_vel, B);
+  PLAUNCH(kb_zero_nnz, (S + 63) / 64, B);
+  TPLAUNCH(TK_ASSEMBLE, kb_ppe_assemble, B);
+  return 0;
+}
+
+// the phases from comp_vel to the closing eddyvis
+static int ph_post_launch(isph_batch* b, const PhaseBatch& B) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const int S = h->S;
+  auto tot = [&](int kind) { return h->h_pre[kind * (S + 1) + S]; };
+  TPLAUNCH(TK_COMPVEL, kb_comp_vel, B);
+  PLAUNCH(kb_cell_max, tot(PH_CELL), B, 0);
+  PLAUNCH(kb_comp_pos, tot(PH_PART), B);
+  int rc = ph_rebin(b, B, 1);
+  if (rc == 0) rc = ph_neighbour(b, B);
+  if (rc) return rc;
+  PLAUNCH(kb_freesurf, tot(PH_PART), B);
+  TPLAUNCH(TK_EFFPOR, kb_effpor, B);
+  TPLAUNCH(TK_SHIFT, kb_opt2_shift, B);
+  PLAUNCH(kb_shift_apply, tot(PH_PART), B);
+  PLAUNCH(kb_densityupdate, tot(PH_PART), B);
+  PLAUNCH(kb_timestep, S, B);
+  TPLAUNCH(TK_EDDYVIS, kb_eddyvis, B);
+  return 0;
+}
+
+// Run a phase sequence directly, or through its graph: captured on first
+// use and again whenever a launch shape (the block totals) changes, then
+// replayed.  The kernels' arguments are the batch's device tables, which
+// keep their addresses; what changes between steps lives in them.
+static int ph_run(isph_batch* b, const PhaseBatch& B, bool pre) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const int S = h->S;
+  if (!h->use_graph) return pre ? ph_pre_launch(b, B) : ph_post_launch(b, B);
+  bool same = true;
+  for (int k = 0; k < PH_NKIND; k++) same = same && h->gkey[k] == h->h_pre[k * (S + 1) + S];
+  if (!same) ph_drop_graphs(h);
+  hipGraphExec_t& ex = pre ? h->gpre : h->gpost;
+  if (!ex) {
+    hipGraph_t g = nullptr;
+    const int64_t l0 = h->launches;
+    HIPCHK(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeRelaxed));
+    int rc = pre ? ph_pre_launch(b, B) : ph_post_launch(b, B);
+    h->gcount[pre ? 0 : 1] = h->launches - l0;
+    hipError_t e = hipStreamEndCapture(c0->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_err(c0, "graph capture: %s", hipGetErrorString(e));
+      return ISPH_EHIP;
+    }
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      ex = nullptr;
+      set_err(c0, "graph instantiation: %s", hipGetErrorString(e));
+      return ISPH_EHIP;
+    }
+    for (int k = 0; k < PH_NKIND; k++) h->gkey[k] = h->h_pre[k * (S + 1) + S];
+  } else {
+    // a replay: the same launches as the capture
+    h->launches += h->gcount[pre ? 0 : 1];
+  }
+  HIPCHK(c0, hipGraphLaunch(ex, c0->stream));
+  return 0;
+}
+
+// the scalars gathered at the start of a batched step: the error bits the
+// previous step's phases left, and the scalar sub-step counts
+static int ph_check(isph_batch* b) {
+  PhaseHost* h = b->ph;
+  for (int s = 0; s < h->S; s++) {
+    isph_ctx* c = b->ctx[s];
+    const DevScalars& d = h->h_sc[s];
+    if (d.err) {
+      int rc = check_err_flags(c);
+      if (rc) return rc;
+    }
+    if (d.solsteps < 1 || d.solsteps > ISPH_MAX_SOLSTEPS) {
+      set_err(c, "scalar sub-step count %d outside [1, %d]", d.solsteps, ISPH_MAX_SOLSTEPS);
+      return ISPH_ENONFINITE;
+    }
+    set_nhi(c, d.n);
+  }
+  return 0;
+}
+
+// One production step of every scenario, the particle phases batched: the
+// order of step_pre, the solve and step_post (main.f90:199-261).
+static int ph_step(isph_batch* b, std::vector<isph_solve_stats>& pst, int* worst) {
+  PhaseHost* h = b->ph;
+  isph_ctx* c0 = b->ctx[0];
+  const int S = h->S;
+  h->launches = 0;
+  int rc = ph_upload_devs(b);
+  if (rc) return rc;
+  rc = ph_gather(b);
+  if (rc) return rc;
+  rc = ph_check(b);
+  if (rc) return rc;
+  int qmax = 1;
+  for (int s = 0; s < S; s++) {
+    isph_ctx* c = b->ctx[s];
+    const DevScalars& d = h->h_sc[s];
+    qmax = std::max(qmax, d.solsteps);
+    double X = 0.0, U = 0.0;
+    // the device's t + dt, the same double sum
+    if (c->wave &&
+        isph_wavemaker_motion(c->wv_stroke, c->wv_period, c->wv_ramp, c->wv_t0, d.t + d.dt,
+                              &X, &U))
+      return ISPH_EARG;
+    h->h_wave[3 * s] = X;
+    h->h_wave[3 * s + 1] = U;
+    h->h_wave[3 * s + 2] = c->wave ? 1.0 : 0.0;
+  }
+  rc = ph_upload(b);
+  if (rc) return rc;
+  const PhaseBatch B{S, h->d_devs, h->d_pre, h->d_wave};
+  auto tot = [&](int kind) { return h->h_pre[kind * (S + 1) + S]; };
+  const int sg = (S + 63) / 64;
+  PLAUNCH(kb_advance_t, sg, B);
+  for (int q = 0; q < qmax; q++) {
+    TPLAUNCH(TK_SCALART, kb_scalart, B, q);
+    PLAUNCH(kb_scalarupdate, tot(PH_PART), B, q);
+  }
+  rc = ph_run(b, B, true);
+  if (rc) return rc;
+  ph_rebin_host(b);
+  for (isph_ctx* c : b->ctx) {
+    c->have_nbr = true;
+    c->assembled = true;
+  }
+  // the solves: the precision-form scenarios together, FP64 ones in turn
+  if (b->pb) {
+    rc = bh_solve(b->pb, pst.data());
+    if (rc) return rc;
+  }
+  size_t q = 0;
+  for (size_t s = 0; s < b->ctx.size(); s++) {
+    isph_ctx* c = b->ctx[s];
+    isph_solve_stats& st = b->last[s];
+    int src = 0;
+    if (c->P.precision != ISPH_PPE_FP64) {
+      st = pst[q++];
+      c->last = st;
+      if (!std::isfinite(st.rel_residual) || !std::isfinite(st.bnorm)) {
+        set_err(c, "non-finite PPE residual");
+        src = ISPH_ENONFINITE;
+      } else if (!st.converged) {
+        set_err(c, "PPE solve did not converge: %d iterations, rel residual %g",
+                st.iterations, st.rel_residual);
+        src = ISPH_ENOCONV;
+      }
+    } else {
+      src = check_err_flags(c);
+      if (src == 0) src = ppe_solve_impl(c, &st);
+    }
+    if (solve_fatal(src) || src == ISPH_ECAP) return src;
+    if (src) *worst = src;
+  }
+  rc = ph_run(b, B, false);
+  if (rc) return rc;
+  ph_rebin_host(b);
+  for (isph_ctx* c : b->ctx) c->have_nbr = true;
+  return 0;
+}
+
+// a scenario the batched phases cover: ISPH (not weakly compressible), no
+// insertion clock on a buffer zone
+static bool ph_coverable(const isph_ctx* c) {
+  if (c->wcsph) return false;
+  for (const HostZone& z : c->zones)
+    if (z.set && z.interval > 0.0) return false;
+  return true;
+}
 
 extern "C" int isph_batch_create(int32_t n_scen, isph_ctx* const* ctxs, isph_batch** out) {
   if (out) *out = nullptr;
@@ -1750,6 +2207,10 @@ extern "C" int isph_batch_create(int32_t n_scen, isph_ctx* const* ctxs, isph_bat
       set_err(ctxs[s], "batched contexts must share one device");
       return ISPH_EARG;
     }
+    if (ctxs[s]->in_batch) {
+      set_err(ctxs[s], "context already belongs to a batch");
+      return ISPH_ESTATE;
+    }
   }
   isph_batch* b = new isph_batch();
   b->ctx.assign(ctxs, ctxs + n_scen);
@@ -1757,15 +2218,28 @@ extern "C" int isph_batch_create(int32_t n_scen, isph_ctx* const* ctxs, isph_bat
     delete b;
     return ISPH_EHIP;
   }
+  bool phased = true;
   for (isph_ctx* c : b->ctx) {
     (void)hipStreamSynchronize(c->stream);
     b->own.push_back(c->stream);
     c->stream = b->stream;
+    c->in_batch = true;
     if (c->P.precision != ISPH_PPE_FP64 && !c->wcsph) b->pctx.push_back(c);
+    phased = phased && ph_coverable(c);
   }
   if (!b->pctx.empty()) {
     int rc = bh_make(b->pctx.data(), (int)b->pctx.size(), &b->pb);
     if (rc) {
+      isph_batch_destroy(b);
+      return rc;
+    }
+  }
+  if (phased) {
+    int rc = ph_make(n_scen, &b->ph);
+    if (rc == 0) rc = ph_upload_devs(b);
+    if (rc == 0) rc = ph_gather(b);  // the live counts before the first step
+    if (rc) {
+      set_err(b->ctx[0], "batch phase tables: allocation or upload failed");
       isph_batch_destroy(b);
       return rc;
     }
@@ -1779,6 +2253,19 @@ extern "C" int isph_batch_step(isph_batch* b, int32_t nsteps) {
   if (!b || nsteps < 0) return ISPH_EARG;
   int worst = 0;
   std::vector<isph_solve_stats> pst(b->pctx.size());
+  if (b->ph) {
+    for (int k = 0; k < nsteps; k++) {
+      int rc = ph_step(b, pst, &worst);
+      if (rc) return rc;
+    }
+    // the last step's error bits
+    int rc = ph_upload_devs(b);
+    if (rc == 0) rc = ph_gather(b);
+    if (rc) return rc;
+    for (int s = 0; s < b->ph->S; s++)
+      if (b->ph->h_sc[s].err && (rc = check_err_flags(b->ctx[s]))) return rc;
+    return worst;
+  }
   for (int k = 0; k < nsteps; k++) {
     for (isph_ctx* c : b->ctx) {
       int rc = step_pre(c);
@@ -1824,6 +2311,50 @@ extern "C" int isph_batch_step(isph_batch* b, int32_t nsteps) {
   return worst;
 }
 
+extern "C" int isph_batch_set_graph(isph_batch* b, int32_t on) {
+  if (!b) return ISPH_EARG;
+  if (!b->ph) {
+    if (on) set_err(b->ctx[0], "the batch's phases run per scenario; no graph to replay");
+    return on ? ISPH_ESTATE : 0;
+  }
+  HIPCHK(b->ctx[0], hipStreamSynchronize(b->stream));
+  ph_drop_graphs(b->ph);
+  b->ph->use_graph = on != 0;
+  return 0;
+}
+
+extern "C" int isph_batch_set_shape(isph_batch* b, int32_t kernel, int32_t shape) {
+  if (!b || kernel < 0 || kernel >= TK_N || shape < 0 || shape >= ts_count(kernel))
+    return ISPH_EARG;
+  if (!b->ph) {
+    set_err(b->ctx[0], "the batch's phases run per scenario; set its contexts' shapes");
+    return ISPH_ESTATE;
+  }
+  HIPCHK(b->ctx[0], hipStreamSynchronize(b->stream));
+  b->ph->tshape[kernel] = shape;
+  ph_drop_graphs(b->ph);  // the launch shapes changed
+  return 0;
+}
+
+// launches of the last batched step (0 when the phases run per scenario)
+extern "C" int64_t isph_batch_launches(isph_batch* b) {
+  return (b && b->ph) ? b->ph->launches : 0;
+}
+
+// live particles of scenario s after the last batched step
+extern "C" int isph_batch_count(isph_batch* b, int32_t s, int64_t* n) {
+  if (!b || !n || s < 0 || s >= (int32_t)b->ctx.size()) return ISPH_EARG;
+  if (b->ph) {
+    *n = b->ph->h_sc[s].n;
+    return 0;
+  }
+  DevScalars d;
+  int rc = read_scalars(b->ctx[s], &d);
+  if (rc) return rc;
+  *n = d.n;
+  return 0;
+}
+
 extern "C" int isph_batch_stats(isph_batch* b, int32_t s, isph_solve_stats* out) {
   if (!b || !out || s < 0 || s >= (int32_t)b->ctx.size()) return ISPH_EARG;
   *out = b->last[s];
@@ -1833,8 +2364,12 @@ extern "C" int isph_batch_stats(isph_batch* b, int32_t s, isph_solve_stats* out)
 extern "C" int isph_batch_destroy(isph_batch* b) {
   if (!b) return 0;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (size_t s = 0; s < b->own.size(); s++) b->ctx[s]->stream = b->own[s];
+  for (size_t s = 0; s < b->own.size(); s++) {
+    b->ctx[s]->stream = b->own[s];
+    b->ctx[s]->in_batch = false;
+  }
   bh_free(b->pb);
+  ph_free(b->ph);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -2068,5 +2603,164 @@ extern "C" int isph_ppe_solve_noalloc(int32_t n, int32_t ld, const int32_t* row_
   if (!out.converged)
     return na_fail(ISPH_ENOCONV, "isph_ppe_solve_noalloc: %d iterations, rel residual %g",
                    out.iterations, out.rel_residual);
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// autotuner: launch shapes of the particle kernels, measured on the state
+// ---------------------------------------------------------------------
+// Mean microseconds per launch of `launch` over `trials` launches after one
+// untimed launch, from HIP events on the stream.
+template <typename F>
+static int tune_time(isph_ctx* c, hipStream_t st, int trials, F launch, double* us) {
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  int rc = launch();
+  if (rc == 0) HIPCHK(c, hipEventRecord(e0, st));
+  for (int k = 0; rc == 0 && k < trials; k++) rc = launch();
+  if (rc == 0) HIPCHK(c, hipEventRecord(e1, st));
+  if (rc == 0) HIPCHK(c, hipEventSynchronize(e1));
+  float ms = 0.f;
+  if (rc == 0) HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *us = 1e3 * ms / trials;
+  return rc;
+}
+
+extern "C" int isph_set_shape(isph_ctx* c, int32_t kernel, int32_t shape) {
+  if (!c || kernel < 0 || kernel >= TK_N || shape < 0 || shape >= ts_count(kernel))
+    return ISPH_EARG;
+  c->tshape[kernel] = shape;
+  return 0;
+}
+
+extern "C" int isph_autotune(isph_ctx* c, int32_t trials, int32_t* shape, double* us) {
+  if (!c || trials < 1) return ISPH_EARG;
+  if (!c->have_nbr || c->wcsph) {
+    set_err(c, "autotune needs an ISPH state at a step boundary (after init_state or a step)");
+    return ISPH_ESTATE;
+  }
+  DevScalars sc;
+  int rc = read_scalars(c, &sc);
+  if (rc) return rc;
+  set_nhi(c, sc.n);
+  Dev& D = c->D;
+  const size_t nb = 8 * (size_t)sc.n;
+  // comp_vel rewrites u and v: keep them in the other half of the ping-pong
+  // state (free until the next rebin writes it) and put them back after
+  HIPCHK(c, hipMemcpyAsync(D.u2, D.u, nb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(D.v2, D.v, nb, hipMemcpyDeviceToDevice, c->stream));
+  for (int tk = 0; tk < TK_N; tk++) {
+    double best = 1e300;
+    int sbest = TS_P256;
+    for (int q = 0; q < TS_N; q++) {
+      if (q >= ts_count(tk)) {
+        if (us) us[TS_N * tk + q] = -1.0;
+        continue;
+      }
+      c->tshape[tk] = q;
+      auto launch = [&]() -> int {
+        switch (tk) {
+          case TK_NEIGHBOUR: return launch_neighbour(c);
+          case TK_EFFPOR: TLAUNCH(tk, k_effpor, c->n_hi, c->D); break;
+          case TK_EDDYVIS: TLAUNCH(tk, k_eddyvis, c->n_hi, c->D); break;
+          case TK_VARTS: TLAUNCH(tk, k_int_vel_varts, c->n_hi, c->D); break;
+          case TK_INTVEL: TLAUNCH(tk, k_int_vel, c->n_hi, c->D); break;
+          case TK_ASSEMBLE: TLAUNCH(tk, k_ppe_assemble, c->n_hi, c->D); break;
+          case TK_COMPVEL: TLAUNCH(tk, k_comp_vel, c->n_hi, c->D); break;
+          case TK_SHIFT: TLAUNCH(tk, k_opt2_shift, c->n_hi, c->D); break;
+          default: TLAUNCH(tk, k_scalart, c->n_hi, c->D); break;
+        }
+        return 0;
+      };
+      double t = 0.0;
+      rc = tune_time(c, c->stream, trials, launch, &t);
+      if (rc) return rc;
+      if (us) us[TS_N * tk + q] = t;
+      if (t < best) {
+        best = t;
+        sbest = q;
+      }
+    }
+    c->tshape[tk] = sbest;
+    if (shape) shape[tk] = sbest;
+  }
+  HIPCHK(c, hipMemcpyAsync(D.u, D.u2, nb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(D.v, D.v2, nb, hipMemcpyDeviceToDevice, c->stream));
+  return check_err_flags(c);
+}
+
+extern "C" int isph_batch_autotune(isph_batch* b, int32_t trials, int32_t* shape, double* us) {
+  if (!b || trials < 1) return ISPH_EARG;
+  isph_ctx* c0 = b->ctx[0];
+  if (!b->ph) {
+    set_err(c0, "the batch's phases run per scenario; tune its contexts instead");
+    return ISPH_ESTATE;
+  }
+  for (isph_ctx* c : b->ctx)
+    if (!c->have_nbr) {
+      set_err(c0, "autotune needs every scenario at a step boundary");
+      return ISPH_ESTATE;
+    }
+  PhaseHost* h = b->ph;
+  int rc = ph_upload_devs(b);
+  if (rc == 0) rc = ph_gather(b);
+  if (rc == 0) rc = ph_check(b);
+  for (int s = 0; rc == 0 && s < h->S; s++) h->h_wave[3 * s + 2] = 0.0;
+  if (rc == 0) rc = ph_upload(b);
+  if (rc) return rc;
+  for (isph_ctx* c : b->ctx) {
+    const size_t nb = 8 * (size_t)c->n_hi;
+    HIPCHK(c0, hipMemcpyAsync(c->D.u2, c->D.u, nb, hipMemcpyDeviceToDevice, c0->stream));
+    HIPCHK(c0, hipMemcpyAsync(c->D.v2, c->D.v, nb, hipMemcpyDeviceToDevice, c0->stream));
+  }
+  const PhaseBatch B{h->S, h->d_devs, h->d_pre, h->d_wave};
+  for (int tk = 0; tk < TK_N; tk++) {
+    double best = 1e300;
+    int sbest = TS_P256;
+    for (int q = 0; q < TS_N; q++) {
+      if (q >= ts_count(tk)) {
+        if (us) us[TS_N * tk + q] = -1.0;
+        continue;
+      }
+      h->tshape[tk] = q;
+      auto launch = [&]() -> int {
+        switch (tk) {
+          case TK_NEIGHBOUR: return ph_neighbour(b, B);
+          case TK_EFFPOR: TPLAUNCH(tk, kb_effpor, B); break;
+          case TK_EDDYVIS: TPLAUNCH(tk, kb_eddyvis, B); break;
+          case TK_VARTS: TPLAUNCH(tk, kb_int_vel_varts, B); break;
+          case TK_INTVEL: TPLAUNCH(tk, kb_int_vel, B); break;
+          case TK_ASSEMBLE: TPLAUNCH(tk, kb_ppe_assemble, B); break;
+          case TK_COMPVEL: TPLAUNCH(tk, kb_comp_vel, B); break;
+          case TK_SHIFT: TPLAUNCH(tk, kb_opt2_shift, B); break;
+          default: TPLAUNCH(tk, kb_scalart, B, 0); break;
+        }
+        return 0;
+      };
+      double t = 0.0;
+      rc = tune_time(c0, c0->stream, trials, launch, &t);
+      if (rc) return rc;
+      if (us) us[TS_N * tk + q] = t;
+      if (t < best) {
+        best = t;
+        sbest = q;
+      }
+    }
+    h->tshape[tk] = sbest;
+    if (shape) shape[tk] = sbest;
+  }
+  for (isph_ctx* c : b->ctx) {
+    const size_t nb = 8 * (size_t)c->n_hi;
+    HIPCHK(c0, hipMemcpyAsync(c->D.u, c->D.u2, nb, hipMemcpyDeviceToDevice, c0->stream));
+    HIPCHK(c0, hipMemcpyAsync(c->D.v, c->D.v2, nb, hipMemcpyDeviceToDevice, c0->stream));
+  }
+  ph_drop_graphs(h);  // the launch shapes changed
+  rc = ph_gather(b);
+  if (rc) return rc;
+  for (int s = 0; s < h->S; s++)
+    if (h->h_sc[s].err && (rc = check_err_flags(b->ctx[s]))) return rc;
   return 0;
 }

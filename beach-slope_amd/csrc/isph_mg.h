@@ -33,7 +33,7 @@ static_assert(MG_CO_THREADS % (2 * MG_DENSE_MAX) == 0,
               "the dense inverse's update maps whole rows of 2 MG_DENSE_MAX columns");
 // The first level of the coarse workgroup has at most MG_CO_PER unknowns per
 // thread: their stencils stay in registers for the whole cycle.
-#define MG_CO_PER 3
+#define MG_CO_PER 1
 // LDS the coarse workgroup may use (bytes), below the device's per-workgroup
 // limit: the vectors x, b, r of its levels and the stencils below its first.
 #define MG_CO_LDS_MAX (144 * 1024)

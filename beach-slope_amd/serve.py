@@ -141,7 +141,7 @@ class Server:
             still = []
             for q, (r, s, k) in enumerate(active):
                 r.iterations.append(batch.stats(q).iterations)
-                self.particle_steps += r.case.n   # particles admitted
+                self.particle_steps += batch.count(q)   # live particles of the step
                 k += 1
                 if k >= r.steps:
                     r.finished = self.clock.now()

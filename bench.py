@@ -2,7 +2,8 @@
 on one MI355X.
 
 A step is one production-loop iteration of the reference (main.f90:199-261)
-over the 650k-particle flume with its piston wavemaker: scalar transport,
+over the 650k-particle flume on its porous beach (128,000 solids) with its
+piston wavemaker: scalar transport,
 projection, the paddle's motion, rebin, neighbour
 search, porosity, SPS, intermediate velocity, PPE assembly and solve,
 pressure-gradient update, trapezoid positions, second rebin and neighbour
@@ -22,13 +23,16 @@ algorithmic bytes per particle-step (bytes_model.step_bytes, at the measured
 iteration count), the rate that gives at the measured speed, and the
 fraction of the 8 TB/s peak.
 
-The roofline object is for the dominant kernel, the solver's SpMV t = A s
-with its two fused dot products, timed with HIP events on the context's
-stream over the timed region.  Its `traffic` (HBM bytes per launch) comes
-from the FETCH_SIZE / WRITE_SIZE counter passes summarised in
-profiles/traffic_flume_650k.md, when that file is present.  The CPU baseline
-is the oracle (a C/OpenMP restatement of the reference) on a bounded sample
-of the same workload.
+The roofline object is for the headline form's dominant kernel: the timed
+solver kernels (the two SpMVs, the V-cycle's level-0 passes and its coarse
+workgroup) are timed with HIP events on the context's stream over the timed
+region, and the one with the most time per step is reported, with its
+algorithmic bytes per launch (bytes_model) over its mean launch time.  Its
+`traffic` (HBM bytes per launch) comes from the FETCH_SIZE / WRITE_SIZE
+counter passes summarised in profiles/traffic_flume_650k.md, when that file
+holds the kernel.  The CPU baseline is the oracle (a C/OpenMP restatement of
+the reference) on a bounded sample of the same workload: one untimed step,
+then three timed steps.
 
 Three more legs run by default, at bounded sizes, and go into the same line:
   precision_errors  the jittered dam break, 200 steps in each form: pressure,
@@ -37,16 +41,21 @@ Three more legs run by default, at bounded sizes, and go into the same line:
            develop in those 200 steps, the same dam break at dx 0.01 m with
            the toe 5 cm beyond the column for 1,250 steps (runup, front and
            slope-surface differences);
-  batched  64 independent flumes, N ~ U[20k, 300k] (about 10M particles),
-           precision form, stepped together (one set of solver launches for
-           all), against the same scenarios stepped one after another alone;
-  served   a seeded Poisson stream (2 requests/s) of flumes with N
-           log-uniform on [10k, 1M], admitted against a 240 GB HBM budget and
-           continuously batched; p50/p99 completion latency, scenarios/h and
-           particle-steps/s.  Steps per request are reduced (default 40).
+  batched  64 independent porous-beach flumes, N ~ U[20k, 300k] (about 10M
+           particles), precision form, stepped together (one launch per
+           particle phase and one set of solver launches for all), 20 steps,
+           against the same scenarios stepped one after another alone;
+  served   a seeded Poisson stream (2 requests/s) of 8 porous-beach flumes
+           with N log-uniform on [10k, 1M], 2,000 steps each, admitted
+           against a 240 GB HBM budget and continuously batched; p50/p99
+           completion latency, scenarios/h and particle-steps/s.
+
+The particle kernels run in the launch shapes the autotuner picks on the
+warmed-up state (isph_autotune; --no-tune keeps the defaults): tuning sits
+outside every timed region, and results do not depend on the shape.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode both|plain|chosen]
-                       [--no-batch] [--no-serve] [--no-cpu]
+                       [--no-batch] [--no-serve] [--no-cpu] [--no-tune]
        python bench.py --copy-calibration   (a 1 GiB device copy, for the
                                              counter calibration only)
 """
@@ -68,13 +77,19 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 650k ISPH; scenarios/h + p99 latency vs plain GPU FP64"
 
 
-def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
+def _shapes(tuned: dict) -> dict:
+    return {k: v[0] for k, v in tuned.items()}
+
+
+def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float, tune: bool):
     params = dict(case.params)
     params.update(precision=precision, tol=tol, max_outer=20)
     g = pkg.Scenario(case, params=params)
     g.init_state()
     for _ in range(warmup):
         g.step(1)
+    # launch shapes of the particle kernels, timed on the warmed-up state
+    shapes = _shapes(g.autotune(trials=3)) if tune else None
     g.prof_enable(True)
     iters, outer = [], []
     nnz = 0.0
@@ -85,16 +100,27 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
         outer.append(st.outer)
         nnz = st.nnz
     t1 = time.perf_counter()
-    kname = "spmv_s32" if precision else "spmv_s"
-    ms, launches = g.prof_read(kname)
+    from beach_slope_amd import bytes_model
     n = g.n
+    timed = {}
+    for kname in (TIMED_PRECISION if precision else TIMED_PLAIN):
+        ms, launches = g.prof_read(kname)
+        kern = KERNEL_OF[kname]
+        if kern == "kb_mg_coarse":
+            kb = bytes_model.coarse_bytes(case.params["cellx"], case.params["celly"])
+        else:
+            kb = bytes_model.kernel_bytes(kern, n, nnz)
+        avg = 1e3 * ms / max(launches, 1)
+        timed[kname] = dict(kernel=kern, ms_per_step=ms / steps, launches=launches,
+                            avg_us=avg, bytes=kb,
+                            gbs=kb / (avg * 1e-6) / 1e9 if launches else 0.0)
     g.prof_enable(False)
     g.close()
     wall = t1 - t0
-    avg_us = 1e3 * ms / max(launches, 1)
-    from beach_slope_amd import bytes_model
-    bytes_per = bytes_model.kernel_bytes(KERNEL_OF[kname], n, nnz)
-    achieved = bytes_per / (avg_us * 1e-6) / 1e9 if launches else 0.0
+    # the dominant kernel: the most time per step
+    kname = max(timed, key=lambda k: timed[k]["ms_per_step"])
+    avg_us, launches = timed[kname]["avg_us"], timed[kname]["launches"]
+    bytes_per, achieved = timed[kname]["bytes"], timed[kname]["gbs"]
     # the whole step against the HBM roofline: algorithmic bytes per step
     # (bytes_model.step_bytes) at the measured rate, over the 8 TB/s peak
     step_b = bytes_model.step_bytes("precision" if precision else "plain", n, nnz,
@@ -106,13 +132,16 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float):
         outer_mean=float(np.mean(outer)), nnz_per_row=nnz / n,
         kernel=kname, kernel_avg_us=avg_us, kernel_launches=launches,
         kernel_bytes=bytes_per, kernel_gbs=achieved,
+        timed_kernels={k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                           for kk, vv in v.items()} for k, v in timed.items()},
+        launch_shapes=shapes,
         step_bytes_per_particle=step_b / n, step_gbs=step_gbs,
         step_hbm_frac=step_gbs / HBM_PEAK_GBS,
         hbm_bound_particle_steps_per_s=n * HBM_PEAK_GBS * 1e9 / step_b,
     )
 
 
-def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int) -> dict:
+def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int, tune: bool) -> dict:
     """The batched configuration: n_scen independent flumes, N ~ U[20k, 300k]
     (seeded), precision form at 1e-8, stepped together; the same scenarios
     stepped one after another alone give the comparison."""
@@ -125,10 +154,12 @@ def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int) -> dict:
     b = pkg.Batch(scen)
     if warmup:
         b.step(warmup)
+    shapes = _shapes(b.autotune(trials=2)) if tune else None
     t0 = time.perf_counter()
     b.step(steps)
     wall = time.perf_counter() - t0
     its = [b.stats(q).iterations for q in range(len(scen))]
+    launches = b.launches()
     b.close()
     n_live = [sc.n for sc in scen]
     for sc in scen:
@@ -139,6 +170,8 @@ def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int) -> dict:
         sc.init_state()
         if warmup:
             sc.step(warmup)
+        if tune:
+            sc.autotune(trials=2)
         t1 = time.perf_counter()
         sc.step(steps)
         alone += time.perf_counter() - t1
@@ -149,15 +182,18 @@ def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int) -> dict:
                 seconds=wall, particle_steps_per_s=ps, alone_particle_steps_per_s=ps_alone,
                 speedup_vs_alone=ps / ps_alone,
                 scenarios_per_h_at_1000_steps=3600.0 * n_scen / (1000.0 * wall / steps),
+                particle_phase_launches_per_step=launches, launch_shapes=shapes,
+                solids=int(sum(c.ns for c in mix)),
                 iterations_last_step_mean=float(np.mean(its)),
                 iterations_last_step_max=int(max(its)))
 
 
 def run_serve(pkg, cases, n_req: int, steps: int, rate: float) -> dict:
-    """The served configuration: a seeded Poisson stream of flume requests,
-    N log-uniform on [10k, 1M], admitted against a 240 GB HBM budget, each
-    kept device resident for its steps (reduced from the configuration's
-    2000 to bound the run), continuous batching between steps."""
+    """The served configuration: a seeded Poisson stream of porous-beach flume
+    requests, N log-uniform on [10k, 1M], admitted against a 240 GB HBM
+    budget, each kept device resident for its steps (the configuration's
+    2,000), continuous batching between steps.  The request count is reduced
+    to bound the run."""
     from beach_slope_amd import serve
     mix = cases.scenario_mix(n_scen=n_req, seed=99, n_min=10_000, n_max=1_000_000,
                              log_uniform=True, steps=steps)
@@ -265,7 +301,11 @@ def run_runup_errors(pkg, cases, steps: int) -> dict:
     return out
 
 
-KERNEL_OF = {"spmv_s": "k_bicg_2", "spmv_s32": "kb_pb_2"}
+KERNEL_OF = {"spmv_p": "k_bicg_1", "spmv_s": "k_bicg_2", "spmv_p32": "kb_pb_1",
+             "spmv_s32": "kb_pb_2", "mg_pre0": "kb_mg_pre0", "mg_post0": "kb_mg_post0",
+             "mg_coarse": "kb_mg_coarse"}
+TIMED_PLAIN = ("spmv_p", "spmv_s")
+TIMED_PRECISION = ("spmv_p32", "spmv_s32", "mg_pre0", "mg_post0", "mg_coarse")
 TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_flume_650k.md"
 
 
@@ -296,18 +336,34 @@ def copy_calibration() -> int:
     return 0
 
 
-def cpu_baseline(oracle, case, tol: float):
-    """The oracle on one step of the same workload (after its init)."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(oracle, case, tol: float, steps: int = 3):
+    """The oracle on the same workload: one untimed step after its init, then
+    `steps` timed steps."""
     params = dict(case.params)
     params.update(precision=0, tol=tol)
     o = oracle.Oracle(case, params=params)
     o.call("init_state")
+    o.step(1)
+    its = []
     t0 = time.perf_counter()
-    rc, st = o.step(1)
+    for _ in range(steps):
+        rc, st = o.step(1)
+        its.append(st.iterations)
     dt = time.perf_counter() - t0
-    return dict(value=o.n / dt, unit="particle-steps/s", cores=oracle.threads(), kind="port",
-                sample=f"1 step of the same {o.n}-particle workload after init "
-                       f"({st.iterations} BiCGStab iterations, {dt:.1f} s)")
+    return dict(value=o.n * steps / dt, unit="particle-steps/s", cores=oracle.threads(),
+                kind="port", cpu=cpu_model(),
+                sample=f"{steps} steps of the same {o.n}-particle workload after one "
+                       f"untimed step ({its} FP64 BiCGStab iterations, {dt:.1f} s)")
 
 
 def main() -> int:
@@ -318,16 +374,18 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["both", "plain", "chosen"])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (no autotuning after warmup)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--error-steps", type=int, default=200)
     ap.add_argument("--runup-steps", type=int, default=1250)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-scenarios", type=int, default=64)
-    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-steps", type=int, default=40)
+    ap.add_argument("--serve-requests", type=int, default=8)
+    ap.add_argument("--serve-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -343,10 +401,12 @@ def main() -> int:
           file=sys.stderr, flush=True)
     res = {}
     if args.mode in ("both", "plain"):
-        res["plain"] = run_gpu(pkg, c, 0, args.steps, args.warmup, args.tol)
+        res["plain"] = run_gpu(pkg, c, 0, args.steps, args.warmup, args.tol,
+                               not args.no_tune)
         print(f"bench: plain {res['plain']}", file=sys.stderr, flush=True)
     if args.mode in ("both", "chosen"):
-        res["chosen"] = run_gpu(pkg, c, 1, args.steps, args.warmup, args.tol)
+        res["chosen"] = run_gpu(pkg, c, 1, args.steps, args.warmup, args.tol,
+                                not args.no_tune)
         print(f"bench: chosen {res['chosen']}", file=sys.stderr, flush=True)
     head = res.get("chosen") or res["plain"]
     batched = served = errors = None
@@ -356,7 +416,8 @@ def main() -> int:
             errors["runup"] = run_runup_errors(pkg, cases, args.runup_steps)
         print(f"bench: precision errors {errors}", file=sys.stderr, flush=True)
     if not args.no_batch:
-        batched = run_batch(pkg, cases, args.batch_scenarios, args.batch_steps, 1)
+        batched = run_batch(pkg, cases, args.batch_scenarios, args.batch_steps, 1,
+                            not args.no_tune)
         print(f"bench: batched {batched}", file=sys.stderr, flush=True)
     if not args.no_serve:
         served = run_serve(pkg, cases, args.serve_requests, args.serve_steps, 2.0)
@@ -378,15 +439,18 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32+f64" if "chosen" in res else "f64",
         "data": "synthetic",
-        "config": {"workload": "flume_650k: 10 m x 0.4 m flume, dx 0.0025, 4000x160 fluid "
-                               "+ 3 wall layers, piston wavemaker (H 0.05 m, T 1.5 s), "
-                               "scalar patch, dt 5e-5",
+        "config": {"workload": "flume_650k: 10 m x 0.4 m flume on a 1:10 porous beach "
+                               "(porosity 0.4, d50 0.01 m, 128,000 solids), dx 0.0025, "
+                               "4000x160 fluid + 3 wall layers, piston wavemaker (H 0.05 m, "
+                               "T 1.5 s), scalar patch, dt 5e-5",
                    "particles": head["n"], "ppe_rel_tol": args.tol,
                    "form": "chosen" if "chosen" in res else "plain"},
         "roofline": {"bound": "hbm", "achieved": head["kernel_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["kernel_gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(head["kernel"]), "kernel": head["kernel"],
-                     "avg_launch_us": head["kernel_avg_us"]},
+                     "traffic": measured_traffic(head["kernel"]),
+                     "kernel": KERNEL_OF[head["kernel"]],
+                     "avg_launch_us": head["kernel_avg_us"],
+                     "ms_per_step": head["timed_kernels"][head["kernel"]]["ms_per_step"]},
         "cpu_baseline": cpu,
         "precision_errors": errors,
         "batched": batched,

@@ -294,16 +294,56 @@ int isph_batch_create(int32_t n_scen, isph_ctx *const *ctxs, isph_batch **out);
 int isph_batch_step(isph_batch *b, int32_t nsteps);
 int isph_batch_stats(isph_batch *b, int32_t s, isph_solve_stats *out);
 int isph_batch_destroy(isph_batch *b);
+/* Kernel launches of the last step of a batch whose particle phases run
+ * batched (one launch per phase over every scenario: every context is an
+ * ISPH one with no insertion clock on a buffer zone), 0 otherwise.  While
+ * a context belongs to a batch its solve form and the weakly compressible
+ * switch are fixed (isph_set_params / isph_set_wcsph return ISPH_ESTATE). */
+int64_t isph_batch_launches(isph_batch *b);
+/* Live particle count of scenario s after the last batched step. */
+int isph_batch_count(isph_batch *b, int32_t s, int64_t *n);
+/* on != 0: the batched phase sequences before and after the solve run as
+ * HIP graphs, captured at their first step (and again when a scenario's
+ * launch shape changes) and replayed after; results are the same.  Only
+ * for batches whose phases run batched (ISPH_ESTATE otherwise). */
+int isph_batch_set_graph(isph_batch *b, int32_t on);
+int isph_batch_autotune(isph_batch *b, int32_t trials, int32_t *shape, double *us);
 
 /* ---- kernel timing (HIP events on the context's stream) ----
- * When enabled, every launch of the solver's two SpMV kernels is bracketed by
+ * When enabled, every launch of the solver's timed kernels is bracketed by
  * a pair of HIP events; isph_prof_read returns the summed event time and the
- * launch count per kernel.  kernel: "spmv_p" (t = A p), "spmv_s" (t = A s)
- * of the FP64 form; "spmv_p32", "spmv_s32" (t1 = A ph, t2 = A sh) of the
- * precision form. */
+ * launch count per kernel (launches after the solve has finished are left
+ * out).  kernel: "spmv_p" (t = A p), "spmv_s" (t = A s) of the FP64 form;
+ * "spmv_p32", "spmv_s32" (t1 = A ph, t2 = A sh), "mg_pre0", "mg_post0" (the
+ * V-cycle's level-0 passes) and "mg_coarse" (its single-workgroup coarse
+ * levels) of the precision form. */
 int isph_prof_enable(isph_ctx *ctx, int32_t on);
 int isph_prof_read(isph_ctx *ctx, const char *kernel, double *total_ms,
                    int64_t *launches);
+
+/* ---- autotuner (no counterpart in the reference) ----
+ * Times the particle kernels below on the context's current state in each
+ * of their launch shapes (trials launches each, HIP events on its stream)
+ * and launches each in its fastest shape from then on.  Shapes: 0, 1, 2 =
+ * one thread per particle in blocks of 64, 128, 256 threads; for the
+ * neighbour search also 3, 4 = one workgroup per tile of 4 (8) cells of a
+ * cell column on 128 (256) threads, the tile's neighbourhood staged in LDS.
+ * Results do not depend on the shape.  Call it at a step boundary (after
+ * isph_init_state or a step): the timed launches leave the state as it was
+ * apart from values every step recomputes before reading them.  shape
+ * (ISPH_TUNE_N, may be NULL) gets the chosen shapes; us (ISPH_TUNE_C *
+ * ISPH_TUNE_N, may be NULL) the mean microseconds per launch of each shape,
+ * -1 where a shape does not apply.  Kernel order: neighbour, effpor,
+ * eddyvis, int_vel_varts, int_vel, ppe_assemble, comp_vel, opt2_shift,
+ * scalart.  isph_batch_autotune does the same for the batched phases of a
+ * batch (one launch over every scenario). */
+#define ISPH_TUNE_N 9
+#define ISPH_TUNE_C 5
+int isph_autotune(isph_ctx *ctx, int32_t trials, int32_t *shape, double *us);
+/* Launch tuned kernel `kernel` (the order above) in shape `shape` from now on
+ * (ISPH_EARG for a shape that does not apply to it). */
+int isph_set_shape(isph_ctx *ctx, int32_t kernel, int32_t shape);
+int isph_batch_set_shape(isph_batch *b, int32_t kernel, int32_t shape);
 
 /* Device memory a context of this capacity needs, in bytes (for admission). */
 int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap,

@@ -105,7 +105,11 @@ typedef struct ostate {
   double *aval, *b, *sol;
   int assembled;
   /* scalars */
-  double t, dt, dtsol, umax, numax, umax_pend, numax_pend;
+  double t, dt, dtsol, umax, numax;
+  /* per cell: max |vel| and max |nu_t| over the cell's slots at the last
+   * comp_vel that found the cell non-empty (dpcell%maxvel / maxeddy,
+   * integrator.f90:831-841); timestep reduces them over every cell */
+  double *cmaxvel, *cmaxeddy;
   int32_t solsteps;
   isph_solve_stats last;
   /* scratch */
@@ -234,6 +238,8 @@ ostate *or_create(const isph_params *P, int64_t cap, int64_t scap) {
   ALLOC(s->srho, scap);
   ALLOC(s->scell, scap);
   ALLOC(s->sstart, s->ncell + 1);
+  ALLOC(s->cmaxvel, s->ncell);
+  ALLOC(s->cmaxeddy, s->ncell);
   s->dt = P->dt;
   s->solsteps = 1;
   s->dtsol = P->dt;
@@ -247,7 +253,8 @@ void or_destroy(ostate *s) {
                   s->varts, s->cdiff, s->vel, s->b, s->sol, s->w1, s->w2, s->w3, s->w4,
                   s->w5, s->w6, s->w7, s->w8, s->coff, s->tid, s->cell, s->flags, s->perm, s->cls, s->ncnt,
                   s->nbr, s->nr, s->klt1, s->klt2, s->alen, s->acol, s->aval,
-                  s->cstart, s->sx, s->sy, s->sm, s->srho, s->scell, s->sstart, s->tden};
+                  s->cstart, s->sx, s->sy, s->sm, s->srho, s->scell, s->sstart, s->tden,
+                  s->cmaxvel, s->cmaxeddy};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
   for (int z = 0; z < 2; z++) {
     ozone *Z = &s->zone[z];
@@ -798,6 +805,9 @@ int or_ppe_assemble(ostate *s) {
         }
       }
     }
+    /* no neighbours: the identity row (p = 0), the val(dia_pt) = 1 that
+     * isph.f90:425-427 leaves commented out; the live code divides 0 by 0 */
+    if (num == 0) diag = 1.0;
     val[dia] = diag; col[dia] = (int32_t)i; wr[dia] = 1;
     double t1 = val[dia];
     rhs = rhs / t1;
@@ -959,10 +969,37 @@ int or_ppe_solve(ostate *s, isph_solve_stats *out) {
 /* ------------------------------------------------------------------ */
 /* comp_vel, comp_pos (integrator.f90:705-883)                          */
 /* ------------------------------------------------------------------ */
+/* per-cell maxima of |vel| and |nu_t| (integrator.f90:831-841): a cell with
+ * particles takes the maxima over its slots, an empty cell keeps its old
+ * values.  vel counts for the non-buffer fluid particles comp_vel updated
+ * (walls: the never-written slot value 0; a buffer particle: 0, where the
+ * reference reads its slot's stale vel).  NaN counts as 0.  init: the
+ * velocities count as 0 and empty cells are cleared. */
+static double nan0(double a) { return (a == a) ? fabs(a) : 0.0; }
+static void cell_maxima(ostate *s, int init) {
+  for (int32_t k = 0; k < s->ncell; k++) {
+    int32_t i0 = s->cstart[k], i1 = s->cstart[k + 1];
+    if (i1 == i0) {
+      if (init) { s->cmaxvel[k] = 0.0; s->cmaxeddy[k] = 0.0; }
+      continue;
+    }
+    double mv = 0.0, mn = 0.0;
+    for (int32_t i = i0; i < i1; i++) {
+      if (!init && s->tid[i] == ISPH_TID_FLUID && !(s->flags[i] & ISPH_F_BUFFER)) {
+        double a = nan0(s->vel[i]);
+        if (a > mv) mv = a;
+      }
+      double b = nan0(s->nut[i]);
+      if (b > mn) mn = b;
+    }
+    s->cmaxvel[k] = mv;
+    s->cmaxeddy[k] = mn;
+  }
+}
+
 void or_comp_vel(ostate *s) {
   const double dt = s->dt;
-  double umax = 0.0, numax = 0.0;
-#pragma omp parallel for schedule(static) reduction(max : umax, numax)
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < s->n; i++) {
     int buf = (s->flags[i] & ISPH_F_BUFFER) != 0;
     if (s->tid[i] == ISPH_TID_FLUID && !buf) {
@@ -995,15 +1032,10 @@ void or_comp_vel(ostate *s) {
       s->u[i] = u;
       s->v[i] = v;
       s->vel[i] = sqrt((u * u) + (v * v)) / ni;
-      double av = fabs(s->vel[i]);
-      if (av > umax) umax = av;
     }
     if (buf) { s->u[i] = s->us[i]; s->v[i] = 0.0; }
-    double an = fabs(s->nut[i]);
-    if (an > numax) numax = an;
   }
-  s->umax_pend = umax;
-  s->numax_pend = numax;
+  cell_maxima(s, 0);
 }
 
 void or_comp_pos(ostate *s) {
@@ -1040,7 +1072,13 @@ void or_opt2_shift(ostate *s) {
   for (int64_t i = 0; i < s->n; i++) {
     if (!(s->tid[i] == ISPH_TID_FLUID && !(s->flags[i] & ISPH_F_BUFFER))) continue;
     int cnt = s->ncnt[i];
-    if (cnt == 0) continue;
+    if (cnt == 0) {
+      /* no neighbour: no shift (the reference stores its zeroed local
+       * shifts only for count > 0, part_shift.f90:32-35, 38-161, and would
+       * add the slot's saved position and intermediate velocity) */
+      s->xs[i] = 0.0; s->ys[i] = 0.0; s->us[i] = 0.0; s->vs[i] = 0.0;
+      continue;
+    }
     const double *cf = &s->coff[4 * i];
     /* the surface normal of part_shift.f90:50-65 is overwritten by zero at
      * part_shift.f90:67-69 before any use; pint is zero as well */
@@ -1116,8 +1154,13 @@ void or_densityupdate(ostate *s) {
 /* timestep, integrator.f90:885-953 */
 void or_timestep(ostate *s) {
   const isph_params *P = &s->P;
-  s->umax = s->umax_pend;
-  s->numax = s->numax_pend;
+  /* maxima over every cell, emptied ones included (integrator.f90:914-924) */
+  s->umax = 0.0;
+  s->numax = 0.0;
+  for (int32_t k = 0; k < s->ncell; k++) {
+    if (s->cmaxvel[k] > s->umax) s->umax = s->cmaxvel[k];
+    if (s->cmaxeddy[k] > s->numax) s->numax = s->cmaxeddy[k];
+  }
   const double dl = P->dl, sig1 = P->sig1;
   if (!P->fixed_dt) {
     double t = s->t, dtn;
@@ -1222,12 +1265,7 @@ int or_init_state(ostate *s) {
   if (rc) return rc;
   or_effpor(s);
   or_eddyvis(s);
-  s->umax_pend = 0.0;
-  s->numax_pend = 0.0;
-  for (int64_t i = 0; i < s->n; i++) {
-    double an = fabs(s->nut[i]);
-    if (an > s->numax_pend) s->numax_pend = an;
-  }
+  cell_maxima(s, 1);
   or_timestep(s);
   return 0;
 }
@@ -1572,7 +1610,8 @@ void *or_array(ostate *s, const char *name) {
       {"cdiff", s->cdiff}, {"vel", s->vel}, {"ncnt", s->ncnt}, {"nbr", s->nbr}, {"nr", s->nr},
       {"klt1", s->klt1}, {"klt2", s->klt2}, {"cstart", s->cstart}, {"alen", s->alen},
       {"acol", s->acol}, {"aval", s->aval}, {"b", s->b}, {"sol", s->sol}, {"sx", s->sx},
-      {"sy", s->sy}, {"sm", s->sm}, {"srho", s->srho}, {"sstart", s->sstart}, {"tden", s->tden}};
+      {"sy", s->sy}, {"sm", s->sm}, {"srho", s->srho}, {"sstart", s->sstart}, {"tden", s->tden},
+      {"cmaxvel", s->cmaxvel}, {"cmaxeddy", s->cmaxeddy}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); k++)
     if (strcmp(t[k].n, name) == 0) return t[k].p;
   return NULL;

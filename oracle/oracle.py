@@ -120,6 +120,7 @@ _DTYPES = {
     "ncnt": (np.int32, 1), "nbr": (np.int32, NBR_CAP), "nr": (np.float64, NBR_CAP),
     "klt1": (np.float64, NBR_CAP), "klt2": (np.float64, NBR_CAP), "coff": (np.float64, 4),
     "alen": (np.int32, 1), "acol": (np.int32, RCAP), "aval": (np.float64, RCAP),
+    "cstart": (np.int32, 1),
 }
 
 PART_FIELDS = ("x", "y", "u", "v", "p", "m", "rho", "rho0", "vol0", "c")

@@ -762,3 +762,103 @@ def test_scalar_transport_is_the_laplacian_in_both_branches(oracle, cases, mutan
         assert scalar_ratio(oracle, cases, L, solids, quadratic=False) < 1e-12
     assert scalar_ratio(oracle, cases, mutant(12), False)[1] > 1.5
     assert scalar_ratio(oracle, cases, mutant(13), True)[1] < 0
+
+
+# comp_vel's per-cell maxima and timestep, integrator.f90:831-841, 914-924 --------
+def test_emptied_cell_keeps_its_velocity_maximum(oracle, cases):
+    """comp_vel refreshes dpcell%maxvel only for cells that hold particles and
+    timestep takes the maximum over every cell, so a cell a fast particle
+    has left still holds its speed.  A lone particle above the water moves
+    at 5 m/s; after it is moved one cell on and slowed to 0.1 m/s, umax
+    stays 5 while its old cell is empty.  Left in place, it gives 0.1."""
+    c = cases.tiny_box(nx=14, ny=8, solids=False, patch=False)
+    dx, h = c.meta["dx"], c.params["h"]
+    top = 3.5 * dx + (c.meta["ny"] + 5.5) * dx     # inside the walls, above the water
+    for k, val in dict(x=3.5 * dx + 9.0 * dx, y=top, u=0.0, v=0.0, p=0.0, m=1000 * dx * dx,
+                       rho=1000.0, rho0=1000.0, vol0=dx * dx, c=-0.5).items():
+        c.parts[k] = np.append(c.parts[k], val)
+    c.parts["tid"] = np.append(c.parts["tid"], np.int32(3)).astype(np.int32)
+    c.parts["flags"] = np.append(c.parts["flags"], np.uint8(0)).astype(np.uint8)
+
+    def run(move):
+        o = oracle.Oracle(c)
+        assert o.call("init_state") == 0
+        d = int(np.argmax(np.where(o.arr("tid") == 3, o.arr("y"), -1.0)))
+        cell0 = int(o.arr("cell")[d])
+        cs = o.cstart()
+        assert cs[cell0 + 1] - cs[cell0] == 1          # alone in its cell
+        for name in ("p", "us", "vs"):
+            o.put(name, 0.0)
+        us = o.arr("us")
+        us[d] = 5.0
+        o.put("us", us)
+        o.call("comp_vel")
+        o.call("timestep")
+        assert o.scalars().umax == 5.0
+        if move:
+            x = o.arr("x")
+            x[d] += 4.0 * h
+            o.put("x", x)
+            o.call("cellshiftalt")
+            assert o.call("neighbour") == 0
+            o.call("effpor")
+            o.call("eddyvis")
+        d = int(np.argmax(np.where(o.arr("tid") == 3, o.arr("y"), -1.0)))
+        for name in ("p", "us", "vs"):
+            o.put(name, 0.0)
+        us = o.arr("us")
+        us[d] = 0.1
+        o.put("us", us)
+        o.call("comp_vel")
+        o.call("timestep")
+        cs = o.cstart()
+        return o.scalars().umax, int(cs[cell0 + 1] - cs[cell0]), o.arr("cmaxvel", c.params["cellx"] * c.params["celly"])[cell0]
+
+    umax, left, kept = run(move=True)
+    assert left == 0 and kept == 5.0 and umax == 5.0
+    umax, left, kept = run(move=False)
+    assert left == 1 and umax == kept == 0.1
+
+
+# an isolated particle, isph.f90:425-440 ------------------------------------------------
+def test_isolated_particle_gets_the_identity_row(oracle, cases):
+    """A fluid particle with no neighbour (a droplet) has an all-zero PPE row;
+    the reference divides it by its zero diagonal (isph.f90:432-440) and
+    leaves the remedy, diagonal 1, commented out (isph.f90:425-427).  Here
+    the row is the identity with rhs 0, so the droplet's pressure is 0 (the
+    free surface's), the rest of the system is the same, and the step goes on:
+    it falls freely, v = v0 + g dt per step."""
+    c = cases.tiny_box(nx=14, ny=8, solids=False, patch=False)
+    dx = c.meta["dx"]
+    for k, val in dict(x=3.5 * dx + 7.0 * dx, y=3.5 * dx + 13.5 * dx, u=0.0, v=0.0, p=0.0,
+                       m=1000 * dx * dx, rho=1000.0, rho0=1000.0, vol0=dx * dx,
+                       c=-0.5).items():
+        c.parts[k] = np.append(c.parts[k], val)
+    c.parts["tid"] = np.append(c.parts["tid"], np.int32(3)).astype(np.int32)
+    c.parts["flags"] = np.append(c.parts["flags"], np.uint8(0)).astype(np.uint8)
+    o = oracle.Oracle(c)
+    assert o.call("init_state") == 0
+    for ph in ("projection", "cellshift"):
+        o.call(ph)
+    assert o.call("neighbour") == 0
+    for ph in ("effpor", "eddyvis", "int_vel"):
+        o.call(ph)
+    assert o.call("ppe_assemble") == 0
+    d = int(np.argmax(np.where(o.arr("tid") == 3, o.arr("y"), -1.0)))
+    assert o.arr("ncnt")[d] == 0
+    ln, col, val, b = o.arr("alen"), o.arr("acol"), o.arr("aval"), o.arr("b")
+    assert ln[d] == 1 and col[d, 0] == d and val[d, 0] == 1.0 and b[d] == 0.0
+    rc, st = o.ppe_solve()
+    assert rc == 0 and st.converged
+    assert o.arr("p")[d] == 0.0
+    o2 = oracle.Oracle(c)
+    assert o2.call("init_state") == 0
+    v0 = None
+    for k in range(3):
+        rc, st = o2.step(1)
+        assert rc == 0
+        y, v, tid = o2.arr("y"), o2.arr("v"), o2.arr("tid")
+        d = int(np.argmax(np.where(tid == 3, y, -1.0)))
+        if v0 is not None:
+            assert abs(v[d] - (v0 + c.params["g"] * c.params["dt"])) < 1e-12
+        v0 = v[d]

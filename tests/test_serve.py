@@ -40,6 +40,9 @@ class _FakeBatch:
     def stats(self, q):
         return types.SimpleNamespace(iterations=7)
 
+    def count(self, q):
+        return self.scen[q].case.n
+
     def close(self):
         pass
 
