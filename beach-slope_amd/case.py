@@ -312,6 +312,9 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
     `Scenario` and the oracle's wrapper apply on upload.  Without it the
     left end is an ordinary three-layer wall.
 
+    The length is rounded to a whole number of lattice spacings dx, so the
+    right wall sits on the fluid lattice.
+
     The beach surface is sharp by default: the layer has porosity `porosity`
     right up to the slope line (the kernel sum smooths n over 2h).  With
     grade_h > 0 the solid volume instead ramps linearly from 0 at the beach
@@ -322,6 +325,11 @@ def porous_flume(dx: float = 0.0025, length: float = 10.0, depth: float = 0.4,
     at 0.3-0.4 m depth that flume diverges within a few steps."""
     params = default_params(dx)
     params.update(dt=dt, por=porosity, d50=d50, tol=1.0e-10)
+    # a whole number of lattice spacings: the right wall then continues the
+    # fluid and bottom lattice one spacing beyond the last fluid column (off
+    # the lattice it would leave a gap of 0.5-1.5 dx and a misaligned bottom
+    # corner, where the pressure solve goes unstable within tens of steps)
+    length = int(round(length / dx)) * dx
     wflags = None
     if wavemaker:
         stroke = piston_stroke(wave_h, wave_t, depth)

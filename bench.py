@@ -39,7 +39,7 @@ Three more legs run by default, at bounded sizes, and go into the same line:
            free-surface, front and runup differences of the precision form
            against the FP64 run (gauges.py); and, since the runup does not
            develop in those 200 steps, the same dam break at dx 0.01 m with
-           the toe 5 cm beyond the column for 1,250 steps (runup, front and
+           the toe 5 cm beyond the column for 2,000 steps (runup, front and
            slope-surface differences);
   batched  64 independent porous-beach flumes, N ~ U[20k, 300k] (about 10M
            particles), precision form, stepped together (one launch per
@@ -266,9 +266,9 @@ def run_runup_errors(pkg, cases, steps: int) -> dict:
     toe 0.5 m beyond the column, out of reach in its 200 steps, so the same
     dam break at dx = 0.01 m with the toe 5 cm beyond the column and seeded
     +-0.01 dx jitter is run `steps` steps in both forms (the front reaches
-    the slope by step ~700).  A run that stops on an isolated particle (a
-    zero PPE diagonal, as the reference's assembly would produce NaN rows)
-    is reported instead of compared."""
+    the slope by step ~700).  A run that stops with an error is reported
+    instead of compared (an isolated fluid particle no longer stops it: its
+    PPE row is the identity)."""
     from beach_slope_amd import gauges
     c = cases.dam_break(dx=0.01, jitter=0.01, toe=0.55)
     m = c.meta
@@ -379,7 +379,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--error-steps", type=int, default=200)
-    ap.add_argument("--runup-steps", type=int, default=1250)
+    ap.add_argument("--runup-steps", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-scenarios", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
