@@ -25,9 +25,11 @@ fraction of the 8 TB/s peak.
 
 The roofline object is for the headline form's dominant kernel: the timed
 solver kernels (the two SpMVs, the V-cycle's level-0 passes and its coarse
-workgroup) are timed with HIP events on the context's stream over the timed
-region, and the one with the most time per step is reported, with its
-algorithmic bytes per launch (bytes_model) over its mean launch time.  Its
+workgroup) are timed with HIP events on the context's stream over as many
+steps again, right after the timed region (the events cost ~10% of a step,
+so the timed region runs without them), and the one with the most time per
+step is reported, with its algorithmic bytes per launch (bytes_model) over
+its mean launch time.  Its
 `traffic` (HBM bytes per launch) comes from the FETCH_SIZE / WRITE_SIZE
 counter passes summarised in profiles/traffic_flume_650k.md, when that file
 holds the kernel.  The CPU baseline is the oracle (a C/OpenMP restatement of
@@ -90,7 +92,8 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float, tune
         g.step(1)
     # launch shapes of the particle kernels, timed on the warmed-up state
     shapes = _shapes(g.autotune(trials=3)) if tune else None
-    g.prof_enable(True)
+    # the timed region runs without the per-kernel HIP-event timers: their
+    # events cost ~10% of a precision-form step at 650k
     iters, outer = [], []
     nnz = 0.0
     t0 = time.perf_counter()
@@ -100,6 +103,12 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float, tune
         outer.append(st.outer)
         nnz = st.nnz
     t1 = time.perf_counter()
+    # kernel durations: as many further steps with the timers on
+    g.prof_enable(True)
+    prof_iters = []
+    for _ in range(steps):
+        _, st = g.step(1)
+        prof_iters.append(st.iterations)
     from beach_slope_amd import bytes_model
     n = g.n
     timed = {}
@@ -130,6 +139,7 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float, tune
         n=n, seconds=wall, ms_per_step=1e3 * wall / steps,
         particle_steps_per_s=n * steps / wall, iters_mean=float(np.mean(iters)),
         outer_mean=float(np.mean(outer)), nnz_per_row=nnz / n,
+        timer_steps_iters_mean=float(np.mean(prof_iters)),
         kernel=kname, kernel_avg_us=avg_us, kernel_launches=launches,
         kernel_bytes=bytes_per, kernel_gbs=achieved,
         timed_kernels={k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
