@@ -30,6 +30,7 @@ struct BatchDev {
   const Dev* devs;      // S scenario tables
   const MgDev* mgs;     // S multigrid tables
   const int* pre;       // BK_NKIND x (S+1) block prefixes
+  int rows;             // row blocks of all scenarios (the BK_ROWS total)
 };
 
 // scenario s and its local block lb of global block gb in index space kind
@@ -48,17 +49,23 @@ __device__ __forceinline__ bool batch_locate(const BatchDev& B, int kind, int gb
   return true;
 }
 
-#define BATCH_ROW(B)                                              \
-  int s_, vb_;                                                    \
-  if (!batch_locate(B, BK_ROWS, row_block(), s_, vb_)) return;    \
+// A batch of one (every single-context solve) skips the prefix table: its
+// tables' addresses then follow from the kernel arguments alone, so the
+// scenario's fields load in the second round of reads.  Row blocks past
+// B.rows return; a cell launch's grid is exactly the table's total, so every
+// cell workgroup is in range.
+#define BATCH_ROW(B)                                                   \
+  int s_ = 0, vb_ = row_block();                                       \
+  if (vb_ >= B.rows) return;                                           \
+  if (B.S > 1 && !batch_locate(B, BK_ROWS, vb_, s_, vb_)) return;      \
   const Dev& D = B.devs[s_];
 
-#define BATCH_CELLS(B, kind)                                      \
-  int s_, lb_;                                                    \
-  if (!batch_locate(B, kind, blockIdx.x, s_, lb_)) return;        \
-  const Dev& D = B.devs[s_];                                      \
-  const MgDev& G = B.mgs[s_];                                     \
-  const int I_ = lb_ * 256 + threadIdx.x;                         \
+#define BATCH_CELLS(B, kind)                                           \
+  int s_ = 0, lb_ = blockIdx.x;                                        \
+  if (B.S > 1 && !batch_locate(B, kind, blockIdx.x, s_, lb_)) return;  \
+  const Dev& D = B.devs[s_];                                 \
+  const MgDev& G = B.mgs[s_];                                \
+  const int I_ = lb_ * 256 + threadIdx.x;                              \
   (void)D;
 
 // ---- setup ----

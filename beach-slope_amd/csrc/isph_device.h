@@ -100,6 +100,15 @@ struct Dev {
   float *r32, *rh32, *pp32, *sv32, *d32;
 };
 
+// A read through the global address space.  A pointer loaded from a device
+// table is generic, and a read through it is a flat load, which waits on both
+// the vector and the scalar counters; through the global space it is a scalar
+// load when uniform and no store precedes it, else a vector load.
+template <typename T>
+__device__ __forceinline__ T ld_global(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 // ---------------------------------------------------------------------
 // Row-block mapping of the solver's row kernels.  Workgroups are dispatched
 // to the 8 XCDs round robin (workgroup b runs on XCD b mod 8) and each XCD

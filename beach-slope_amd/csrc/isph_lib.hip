@@ -955,7 +955,7 @@ static int bh_refresh(BatchHost* b) {
 
 static int bh_gather(BatchHost* b) {
   isph_ctx* c = b->ctx[0];
-  BatchDev B{b->S, b->d_devs, b->d_mgs, b->d_pre};
+  BatchDev B{b->S, b->d_devs, b->d_mgs, b->d_pre, 0};
   hipLaunchKernelGGL(kb_gather_scalars, dim3((b->S + 63) / 64), dim3(64), 0, c->stream, B, b->d_sc);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(b->sc.data(), b->d_sc, sizeof(DevScalars) * b->S,
@@ -1049,8 +1049,8 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   const int S = b->S;
   int rc = bh_refresh(b);
   if (rc) return rc;
-  const BatchDev B{S, b->d_devs, b->d_mgs, b->d_pre};
   auto tot = [&](int kind) { return b->pre[(size_t)kind * (S + 1) + S]; };
+  const BatchDev B{S, b->d_devs, b->d_mgs, b->d_pre, tot(BK_ROWS)};
   const int grow = 8 * ((tot(BK_ROWS) + 7) / 8);
   // multigrid hierarchy from the assembled rows
   BLAUNCH(kb_mg_setup0, grow, 256, B);

@@ -346,7 +346,7 @@ template <bool OFF, typename VT>
 __device__ __forceinline__ void mg_pre0_row(const Dev& D, const MgDev& G,
                                             const VT* __restrict__ v, float* __restrict__ r0,
                                             int i) {
-  if (*G.done || i >= D.sc->n) return;
+  if ((ld_global(G.done) != 0) | (i >= ld_global(&D.sc->n))) return;  // both reads in one trip
   const float* __restrict__ x0 = G.x0;
   const float acc = ell_row_sum_h<OFF, ELL_UP>(D, i,
                                               [&](int j) { return x0[j]; });
@@ -361,10 +361,16 @@ __device__ __forceinline__ void mg_scale0(const MgDev& G, float v, int i) {
 // level 1, cell I: r1 = R r0, the sum of the cell's contiguous row range
 __device__ __forceinline__ void mg_restrict0_cell(const Dev& D, const MgDev& G,
                                                   const float* __restrict__ r0, int I) {
+  // the done flag is tested before the store, so its read need not precede
+  // the row reads
   const MgLevel L = G.lev[1];
-  if (*G.done || I >= L.n) return;
+  const int* done = G.done;
+  float* bb = G.bb;
+  const int32_t* cstart = D.cstart;
+  if (I >= L.n) return;
   float s = 0.0f;
-  const int i0 = D.cstart[I], i1 = D.cstart[I + 1];
+  const int i0 = cstart[I], i1 = cstart[I + 1];
+  const int dn = ld_global(done);
   for (int b0 = i0; b0 < i1; b0 += 8) {  // eight loads in flight, then the sums in row order
     float v[8];
 #pragma unroll
@@ -373,17 +379,18 @@ __device__ __forceinline__ void mg_restrict0_cell(const Dev& D, const MgDev& G,
     for (int u = 0; u < 8; u++)
       if (b0 + u < i1) s = s + v[u];
   }
-  G.bb[L.off + I] = s;
+  if (dn) return;
+  bb[L.off + I] = s;
 }
 
 // level 0 rows of level-1 cell I: x1 = x0 + c e1, in place of x0 (e1 is the
 // cell's final level-1 correction); run by the pass that forms e1.  Lane q
 // of `lanes` lanes sharing the cell takes rows i0 + q, i0 + q + lanes, ...
-__device__ __forceinline__ void mg_prolong_cell(const Dev& D, const MgDev& G, int I, float e1,
-                                                int q = 0, int lanes = 1) {
-  const float ce = G.cscale * e1;
-  float* __restrict__ x0 = G.x0;
-  const int i0 = D.cstart[I] + q, i1 = D.cstart[I + 1];
+__device__ __forceinline__ void mg_prolong_rows(const int32_t* cstart, float* __restrict__ x0,
+                                                float cscale, int I, float e1, int q,
+                                                int lanes) {
+  const float ce = cscale * e1;
+  const int i0 = cstart[I] + q, i1 = cstart[I + 1];
   for (int b0 = i0; b0 < i1; b0 += 8 * lanes) {
     float v[8];
 #pragma unroll
@@ -403,30 +410,46 @@ __device__ __forceinline__ void mg_prolong_cell(const Dev& D, const MgDev& G, in
 // workgroup pass below, so a level gives the same values whichever pass runs
 // it.  The four lanes of an aggregate share a wavefront and leave together.
 __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
-  if (*G.done || l > G.nmb) return;
+  // lev[l + 1] is in the table for every launched l (l <= nmb < nlev <
+  // MG_MAX_LEVELS); the done flag is tested before the first store, so its
+  // read need not precede the cell reads
+  const int nmb = G.nmb;
   const MgLevel L = G.lev[l], C = G.lev[l + 1];
+  const float w = G.w;
+  const int* done = G.done;
+  float *sten = G.sten, *rdg = G.rdg, *bb = G.bb, *xb = G.xb;
+  if (l > nmb) return;
   const int J = g >> 2, q = g & 3;
   if (J >= C.n) return;
-  const float w = G.w;
-  const float* st = G.sten + 9 * L.off;
-  const float* rd = G.rdg + L.off;
-  const float* b = G.bb + L.off;
-  float* x = G.xb + L.off;
+  const float* st = sten + 9 * L.off;
+  const float* rd = rdg + L.off;
+  const float* b = bb + L.off;
+  float* x = xb + L.off;
   const int CJ = J / C.ny, CI = J - CJ * C.ny;
   const int ci = 2 * CI + (q & 1), cj = 2 * CJ + (q >> 1);
   const bool in = ci < L.ny && cj < L.nx;
-  float res = 0.0f;
-  if (in) {
-    const int I = cj * L.ny + ci;
-    float acc = 0.0f;
-    int Nb[9];
-    mg_nbr9(L, ci, cj, I, Nb);
+  // a child outside the level's grid (odd grid sizes) reads cell 0 and keeps
+  // nothing, so every lane issues its reads together, before the first sum
+  // (a neighbour outside the grid reads the cell itself, and its term is
+  // not added)
+  const int I = in ? cj * L.ny + ci : 0;
+  int Nb[9];
+  mg_nbr9(L, in ? ci : 0, in ? cj : 0, I, Nb);
+  float tk[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++)
-      if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * ((w * b[Nb[k]]) * rd[Nb[k]]);
-    x[I] = (w * b[I]) * rd[I];
-    res = b[I] - acc;
+  for (int k = 0; k < 9; k++) {
+    const int j = Nb[k] >= 0 ? Nb[k] : I;
+    tk[k] = st[k * L.n + I] * ((w * b[j]) * rd[j]);
   }
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+    if (Nb[k] >= 0) acc = acc + tk[k];
+  const float xi = (w * b[I]) * rd[I];
+  const float res = in ? b[I] - acc : 0.0f;
+  const int dn = ld_global(done);
+  if (dn) return;  // the whole grid: the flag is per scenario
+  if (in) x[I] = xi;
   const int lane = threadIdx.x & 63, base = lane & ~3;
   const float r1 = __shfl(res, base + 1, 64), r2 = __shfl(res, base + 2, 64),
               r3 = __shfl(res, base + 3, 64);
@@ -438,7 +461,7 @@ __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
   if (in_mask & 1) s = s + r1;
   if (in_mask & 2) s = s + r2;
   if (in_mask & 4) s = s + r3;
-  G.bb[C.off + J] = s;
+  bb[C.off + J] = s;
 }
 
 // Cell level l (1 <= l <= nmb), up pass, cell I: adds the prolonged
@@ -449,32 +472,49 @@ __device__ __forceinline__ void mg_lvl_down(const MgDev& G, int l, int g) {
 // reads it.  On level 1 four lanes share a cell (g = 4 I + q): each forms
 // the same result and adds it to a quarter of the x0 of the cell's rows.
 __device__ __forceinline__ void mg_lvl_up(const Dev& D, const MgDev& G, int l, int g) {
-  if (*G.done || l > G.nmb) return;
+  // lev[l + 1] as in the down pass; the done flag is tested before the
+  // first store, so its read need not precede the cell reads
+  const int nmb = G.nmb;
   const MgLevel L = G.lev[l], C = G.lev[l + 1];
+  const float w = G.w, c = G.cscale;
+  const int* done = G.done;
+  float *sten = G.sten, *bb = G.bb, *xb = G.xb, *rb = G.rb, *rdg = G.rdg, *x0 = G.x0;
+  const int32_t* cstart = D.cstart;
+  if (l > nmb) return;
   const int I = (l == 1) ? (g >> 2) : g;
   if (I >= L.n) return;
-  const float w = G.w, c = G.cscale;
-  const float* st = G.sten + 9 * L.off;
-  const float* b = G.bb + L.off;
-  const float* x = G.xb + L.off;
-  const float* __restrict__ xc = (l == G.nmb ? G.xb : G.rb) + C.off;
+  const float* st = sten + 9 * L.off;
+  const float* b = bb + L.off;
+  const float* x = xb + L.off;
+  const float* __restrict__ xc = (l == nmb ? xb : rb) + C.off;
   const int cj = I / L.ny, ci = I - cj * L.ny;
+  const int pI = (cj >> 1) * C.ny + (ci >> 1);
   float acc = 0.0f;
   int Nb[9], Pa[9];
   mg_nbr9(L, ci, cj, I, Nb);
   mg_par9(C, ci, cj, Pa);
+  float tk[9];
+  // every load is issued before the first sum (a neighbour outside the grid
+  // reads the cell and its parent, and its term is not added)
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const bool has = Nb[k] >= 0;
+    tk[k] = st[k * L.n + I] * (x[has ? Nb[k] : I] + c * xc[has ? Pa[k] : pI]);
+  }
 #pragma unroll
   for (int k = 0; k < 9; k++)
-    if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (x[Nb[k]] + c * xc[Pa[k]]);
-  const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
-  const float out = x1 + (w * (b[I] - acc)) * G.rdg[L.off + I];
+    if (Nb[k] >= 0) acc = acc + tk[k];
+  const float x1 = x[I] + c * xc[pI];
+  const float out = x1 + (w * (b[I] - acc)) * rdg[L.off + I];
+  const int dn = ld_global(done);
+  if (dn) return;  // the whole grid: the flag is per scenario
   if (l != 1) {
-    G.rb[L.off + I] = out;
+    rb[L.off + I] = out;
     return;
   }
   const int q = g & 3;
-  if (q == 0) G.rb[L.off + I] = out;
-  mg_prolong_cell(D, G, I, out, q, 4);
+  if (q == 0) rb[L.off + I] = out;
+  mg_prolong_rows(cstart, x0, c, I, out, q, 4);
 }
 
 // levels l0 = nmb+1..nlev in one workgroup: down sweeps, dense solve, up
@@ -589,9 +629,12 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
           float acc = 0.0f;
           int Nb[9];
           mg_nbr9(L, ci0[q], cj0[q], I, Nb);
+          float tk[9];
+#pragma unroll
+          for (int k = 0; k < 9; k++) tk[k] = st0[q][k] * x[Nb[k] >= 0 ? Nb[k] : I];
 #pragma unroll
           for (int k = 0; k < 9; k++)
-            if (Nb[k] >= 0) acc = acc + st0[q][k] * x[Nb[k]];
+            if (Nb[k] >= 0) acc = acc + tk[k];
           r[I] = b[I] - acc;
         }
       }
@@ -605,9 +648,12 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
         float acc = 0.0f;
         int Nb[9];
         mg_nbr9(L, ci, cj, I, Nb);
+        float tk[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) tk[k] = st[k * L.n + I] * x[Nb[k] >= 0 ? Nb[k] : I];
 #pragma unroll
         for (int k = 0; k < 9; k++)
-          if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * x[Nb[k]];
+          if (Nb[k] >= 0) acc = acc + tk[k];
         r[I] = b[I] - acc;
       }
     }
@@ -657,10 +703,17 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
           int Nb[9], Pa[9];
           mg_nbr9(L, ci, cj, I, Nb);
           mg_par9(C, ci, cj, Pa);
+          const int pI = (cj >> 1) * C.ny + (ci >> 1);
+          float tk[9];
+#pragma unroll
+          for (int k = 0; k < 9; k++) {
+            const bool has = Nb[k] >= 0;
+            tk[k] = st0[q][k] * (x[has ? Nb[k] : I] + c * xc[has ? Pa[k] : pI]);
+          }
 #pragma unroll
           for (int k = 0; k < 9; k++)
-            if (Nb[k] >= 0) acc = acc + st0[q][k] * (x[Nb[k]] + c * xc[Pa[k]]);
-          const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+            if (Nb[k] >= 0) acc = acc + tk[k];
+          const float x1 = x[I] + c * xc[pI];
           out[I] = x1 + (w * (b[I] - acc)) * rd0[q];
         }
       }
@@ -671,10 +724,17 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
         int Nb[9], Pa[9];
         mg_nbr9(L, ci, cj, I, Nb);
         mg_par9(C, ci, cj, Pa);
+        const int pI = (cj >> 1) * C.ny + (ci >> 1);
+        float tk[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const bool has = Nb[k] >= 0;
+          tk[k] = st[k * L.n + I] * (x[has ? Nb[k] : I] + c * xc[has ? Pa[k] : pI]);
+        }
 #pragma unroll
         for (int k = 0; k < 9; k++)
-          if (Nb[k] >= 0) acc = acc + st[k * L.n + I] * (x[Nb[k]] + c * xc[Pa[k]]);
-        const float x1 = x[I] + c * xc[(cj >> 1) * C.ny + (ci >> 1)];
+          if (Nb[k] >= 0) acc = acc + tk[k];
+        const float x1 = x[I] + c * xc[pI];
         out[I] = x1 + (w * (b[I] - acc)) * RD[(L.off - stb) + I];
       }
     }
@@ -685,7 +745,7 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
   const float* res = (l0 == nl ? X : R);
   for (int I = t; I < L0.n; I += T) {
     G.xb[L0.off + I] = res[I];
-    if (l0 == 1) mg_prolong_cell(D, G, I, res[I]);
+    if (l0 == 1) mg_prolong_rows(D.cstart, G.x0, G.cscale, I, res[I], 0, 1);
   }
 }
 
@@ -694,7 +754,7 @@ template <bool OFF, typename VT>
 __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
                                              const VT* __restrict__ v, float* __restrict__ out,
                                              int i) {
-  if (*G.done || i >= D.sc->n) return;
+  if ((ld_global(G.done) != 0) | (i >= ld_global(&D.sc->n))) return;  // both reads in one trip
   const float* __restrict__ x1 = G.x0;
   const float acc = ell_row_sum_h<OFF, ELL_UP>(D, i,
                                               [&](int j) { return x1[j]; });
@@ -703,6 +763,8 @@ __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
 
 // ---------------------------------------------------------------------
 // preconditioned mixed-precision BiCGStab, row bodies (row block vb).
+// The scalars are read through the global space (scalar loads, one round
+// trip for the flag, the live count and the coefficients).
 // ph = M(p) lives in r32, sh = M(s) in sv32 (FP32 directions).  The outer
 // products t1 = A ph and t2 = A sh take the FP64 matrix values and sum in
 // FP64, so the recursive residual follows the true one (b - A x changes by
@@ -714,9 +776,9 @@ __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
 // t1 = A ph (FP64) ; partial (t1 . rh)
 template <bool OFF>
 __device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
-  if (D.sc->done) return;
+  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
+  if (dn) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
-  const int n = D.sc->n;
   double prod = 0.0;
   if (i < n) {
     const float* __restrict__ ph = D.r32;
@@ -731,10 +793,11 @@ __device__ __forceinline__ void pb_1_rows(const Dev& D, int vb, double* lds) {
 
 // s = r - alpha t1 (FP64)
 __device__ __forceinline__ void pb_s_rows(const Dev& D, const MgDev& G, int vb) {
-  if (D.sc->done) return;
+  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
+  const double alpha = ld_global(&D.sc->alpha);
   const int i = vb * RED_BLOCK + threadIdx.x;
-  if (i >= D.sc->n) return;
-  const double sv = D.r[i] - D.sc->alpha * D.t1[i];
+  if ((dn != 0) | (i >= n)) return;  // both reads in one trip
+  const double sv = D.r[i] - alpha * D.t1[i];
   D.sv[i] = sv;
   mg_scale0(G, (float)sv, i);
 }
@@ -742,9 +805,9 @@ __device__ __forceinline__ void pb_s_rows(const Dev& D, const MgDev& G, int vb) 
 // t2 = A sh (FP64) ; partials (t2 . s), (t2 . t2)
 template <bool OFF>
 __device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
-  if (D.sc->done) return;
+  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
+  if (dn) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
-  const int n = D.sc->n;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
     const float* __restrict__ sh = D.sv32;
@@ -764,10 +827,10 @@ __device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
 
 // x += alpha ph + omega sh (FP64) ; r = s - omega t2 ; partials (r . rh), (r . r)
 __device__ __forceinline__ void pb_3_rows(const Dev& D, int vb, double* lds) {
-  if (D.sc->done) return;
+  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
+  const double alpha = ld_global(&D.sc->alpha), omega = ld_global(&D.sc->omega);
+  if (dn) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
-  const int n = D.sc->n;
-  const double alpha = D.sc->alpha, omega = D.sc->omega;
   double p1 = 0.0, p2 = 0.0;
   if (i < n) {
     D.sol[i] = (D.sol[i] + alpha * (double)D.r32[i]) + omega * (double)D.sv32[i];
@@ -786,10 +849,10 @@ __device__ __forceinline__ void pb_3_rows(const Dev& D, int vb, double* lds) {
 
 // p = r + beta (p - omega t1)  (p stored FP32)
 __device__ __forceinline__ void pb_4_rows(const Dev& D, const MgDev& G, int vb) {
-  if (D.sc->done) return;
+  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
+  const double beta = ld_global(&D.sc->beta), omega = ld_global(&D.sc->omega);
   const int i = vb * RED_BLOCK + threadIdx.x;
-  if (i >= D.sc->n) return;
-  const double beta = D.sc->beta, omega = D.sc->omega;
+  if ((dn != 0) | (i >= n)) return;  // both reads in one trip
   const float p = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * D.t1[i]));
   D.pp32[i] = p;
   mg_scale0(G, p, i);
