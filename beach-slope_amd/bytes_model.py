@@ -21,6 +21,16 @@ the 16-bit offset j - i in the precision form) and the FP64 value.
                (9 FP32) and 1/diagonal, the first level's right-hand side
                and the dense FP64 inverse; writes the first level's result
                (coarse_bytes)
+  kb_mg_restrict0  level-1 right-hand side: reads r0 and the cell starts,
+               writes b1
+  kb_mg_down   one multi-workgroup cell level (cell_pass_bytes, averaged over
+               the levels): reads the stencil (9 FP32), b and 1/diagonal,
+               writes x and the parent level's b
+  kb_mg_up     one multi-workgroup cell level: reads the stencil, b, x,
+               1/diagonal and the parent's correction, writes the result; on
+               level 1 also x0 of every row, read and written (prolongation)
+  kb_pfinalize one workgroup folds the row blocks' FP64 partials (one array
+               for alpha, two for omega and beta)
 """
 from __future__ import annotations
 
@@ -59,6 +69,27 @@ def coarse_bytes(cellx: int, celly: int) -> float:
     k = next(i for i, n in enumerate(lv) if n <= MG_CO_UNKNOWNS)
     tail = lv[k:]
     return 4.0 * 10 * sum(tail[:-1]) + 4.0 * 2 * tail[0] + 8.0 * tail[-1] ** 2
+
+
+def cell_pass_bytes(kernel: str, n: int, cellx: int, celly: int) -> float:
+    """Algorithmic bytes of one launch of a cell-level kernel of the V-cycle
+    at n rows: kb_mg_restrict0, or kb_mg_down / kb_mg_up averaged over the
+    multi-workgroup levels (those before the coarse workgroup's first level),
+    or kb_pfinalize averaged over its three launches per iteration."""
+    lv = cell_levels(cellx, celly)
+    k = next(i for i, m in enumerate(lv) if m <= MG_CO_UNKNOWNS)
+    if kernel == "kb_mg_restrict0":
+        return 4.0 * n + 4.0 * (lv[0] + 1) + 4.0 * lv[0]
+    if kernel == "kb_pfinalize":
+        return 8.0 * ((n + 255) // 256) * (1 + 2 + 2) / 3.0
+    if k == 0:
+        return 0.0
+    if kernel == "kb_mg_down":
+        return sum(48.0 * lv[i] + 4.0 * lv[i + 1] for i in range(k)) / k
+    if kernel == "kb_mg_up":
+        tot = sum(52.0 * lv[i] + 4.0 * lv[i + 1] for i in range(k))
+        return (tot + 8.0 * n + 4.0 * (lv[0] + 1)) / k
+    raise KeyError(kernel)
 
 
 def kernel_bytes(kernel: str, n: int, nnz: float) -> float:

@@ -71,8 +71,11 @@ struct isph_ctx {
   bool wave = false;
   double wv_stroke = 0.0, wv_period = 1.0, wv_ramp = 0.0, wv_t0 = 0.0;
   bool prof = false;
-  KTimer timers[7] = {{"spmv_p", 1},    {"spmv_s", 1},    {"spmv_p32", 1}, {"spmv_s32", 1},
-                      {"mg_coarse", 2}, {"mg_pre0", 2},   {"mg_post0", 2}};
+  // per_iter of mg_down and mg_up (2 x the multi-workgroup cell levels) is
+  // set by each solve
+  KTimer timers[11] = {{"spmv_p", 1},    {"spmv_s", 1},  {"spmv_p32", 1},     {"spmv_s32", 1},
+                       {"mg_coarse", 2}, {"mg_pre0", 2}, {"mg_post0", 2},     {"mg_down", 0},
+                       {"mg_up", 0},     {"pfinalize", 3}, {"mg_restrict0", 2}};
   Dev D;
   hipStream_t stream = nullptr;
   int64_t cap = 0, scap = 0;
@@ -1004,12 +1007,12 @@ static int bh_vcycle(BatchHost* b, const BatchDev& B, int grow) {
   const int S = b->S;
   auto tot = [&](int kind) { return b->pre[(size_t)kind * (S + 1) + S]; };
   TIMED_BLAUNCH(5, kb_mg_pre0<V>, grow, B);
-  BLAUNCH(kb_mg_restrict0, tot(BK_CELL1), 256, B);
+  TIMED_BLAUNCH(10, kb_mg_restrict0, tot(BK_CELL1), B);
   for (int l = 1; l <= b->max_nmb; l++)
-    if (tot(BK_DOWN + l - 1) > 0) BLAUNCH(kb_mg_down, tot(BK_DOWN + l - 1), 256, B, l);
+    if (tot(BK_DOWN + l - 1) > 0) TIMED_BLAUNCH(7, kb_mg_down, tot(BK_DOWN + l - 1), B, l);
   TIMED_BLAUNCH_LDS(4, kb_mg_coarse, S, MG_CO_THREADS, b->co_lds, B);
   for (int l = b->max_nmb; l >= 1; l--)
-    if (tot(BK_UP + l - 1) > 0) BLAUNCH(kb_mg_up, tot(BK_UP + l - 1), 256, B, l);
+    if (tot(BK_UP + l - 1) > 0) TIMED_BLAUNCH(8, kb_mg_up, tot(BK_UP + l - 1), B, l);
   TIMED_BLAUNCH(6, kb_mg_post0<V>, grow, B);
   return 0;
 }
@@ -1024,14 +1027,14 @@ static int bh_iterate(BatchHost* b, const BatchDev& B, int grow, int chunk) {
     rc = bh_vcycle<0, OFF>(b, B, grow);
     if (rc) return rc;
     TIMED_BLAUNCH(2, kb_pb_1<OFF>, grow, B);
-    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_ALPHA, 0);
+    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_ALPHA, 0);
     BLAUNCH(kb_pb_s, grow, 256, B);
     rc = bh_vcycle<1, OFF>(b, B, grow);
     if (rc) return rc;
     TIMED_BLAUNCH(3, kb_pb_2<OFF>, grow, B);
-    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_OMEGA, 0);
+    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_OMEGA, 0);
     BLAUNCH(kb_pb_3, grow, 256, B);
-    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BETA, 0);
+    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_BETA, 0);
     BLAUNCH(kb_pb_4, grow, 256, B);
   }
   return 0;
@@ -1051,6 +1054,7 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   if (rc) return rc;
   auto tot = [&](int kind) { return b->pre[(size_t)kind * (S + 1) + S]; };
   const BatchDev B{S, b->d_devs, b->d_mgs, b->d_pre, tot(BK_ROWS)};
+  c->timers[7].per_iter = c->timers[8].per_iter = 2 * b->max_nmb;  // mg_down, mg_up
   const int grow = 8 * ((tot(BK_ROWS) + 7) / 8);
   // multigrid hierarchy from the assembled rows
   BLAUNCH(kb_mg_setup0, grow, 256, B);

@@ -23,13 +23,16 @@ algorithmic bytes per particle-step (bytes_model.step_bytes, at the measured
 iteration count), the rate that gives at the measured speed, and the
 fraction of the 8 TB/s peak.
 
-The roofline object is for the headline form's dominant kernel: the timed
-solver kernels (the two SpMVs, the V-cycle's level-0 passes and its coarse
-workgroup) are timed with HIP events on the context's stream over as many
-steps again, right after the timed region (the events cost ~10% of a step,
-so the timed region runs without them), and the one with the most time per
-step is reported, with its algorithmic bytes per launch (bytes_model) over
-its mean launch time.  Its
+The roofline object is for the headline form's dominant kernel: the
+solver's kernels (the two SpMVs, the V-cycle's level-0 passes, level-1
+restriction, cell-level down and up passes and coarse workgroup, and the
+dot-product finalizes; together ~90% of a step's kernel time) are timed with
+HIP events on the context's stream over as many steps again, right after
+the timed region (the events cost ~10% of a step, so the timed region runs
+without them), and the kernel function with the most time per step (all
+its launches: both applications of the V-cycle, every cell level) is
+reported, with its algorithmic bytes per launch (bytes_model) over its mean
+launch time.  Its
 `traffic` (HBM bytes per launch) comes from the FETCH_SIZE / WRITE_SIZE
 counter passes summarised in profiles/traffic_flume_650k.md, when that file
 holds the kernel.  The CPU baseline is the oracle (a C/OpenMP restatement of
@@ -117,6 +120,8 @@ def run_gpu(pkg, case, precision: int, steps: int, warmup: int, tol: float, tune
         kern = KERNEL_OF[kname]
         if kern == "kb_mg_coarse":
             kb = bytes_model.coarse_bytes(case.params["cellx"], case.params["celly"])
+        elif kern in CELL_PASSES:
+            kb = bytes_model.cell_pass_bytes(kern, n, case.params["cellx"], case.params["celly"])
         else:
             kb = bytes_model.kernel_bytes(kern, n, nnz)
         avg = 1e3 * ms / max(launches, 1)
@@ -313,9 +318,12 @@ def run_runup_errors(pkg, cases, steps: int) -> dict:
 
 KERNEL_OF = {"spmv_p": "k_bicg_1", "spmv_s": "k_bicg_2", "spmv_p32": "kb_pb_1",
              "spmv_s32": "kb_pb_2", "mg_pre0": "kb_mg_pre0", "mg_post0": "kb_mg_post0",
-             "mg_coarse": "kb_mg_coarse"}
+             "mg_coarse": "kb_mg_coarse", "mg_restrict0": "kb_mg_restrict0",
+             "mg_down": "kb_mg_down", "mg_up": "kb_mg_up", "pfinalize": "kb_pfinalize"}
 TIMED_PLAIN = ("spmv_p", "spmv_s")
-TIMED_PRECISION = ("spmv_p32", "spmv_s32", "mg_pre0", "mg_post0", "mg_coarse")
+TIMED_PRECISION = ("spmv_p32", "spmv_s32", "mg_pre0", "mg_post0", "mg_restrict0", "mg_down",
+                   "mg_coarse", "mg_up", "pfinalize")
+CELL_PASSES = ("kb_mg_restrict0", "kb_mg_down", "kb_mg_up", "kb_pfinalize")
 TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_flume_650k.md"
 
 

@@ -315,8 +315,10 @@ int isph_batch_autotune(isph_batch *b, int32_t trials, int32_t *shape, double *u
  * launch count per kernel (launches after the solve has finished are left
  * out).  kernel: "spmv_p" (t = A p), "spmv_s" (t = A s) of the FP64 form;
  * "spmv_p32", "spmv_s32" (t1 = A ph, t2 = A sh), "mg_pre0", "mg_post0" (the
- * V-cycle's level-0 passes) and "mg_coarse" (its single-workgroup coarse
- * levels) of the precision form. */
+ * V-cycle's level-0 passes), "mg_restrict0" (the restriction to level 1),
+ * "mg_down", "mg_up" (the multi-workgroup cell levels, every level),
+ * "mg_coarse" (its single-workgroup coarse levels) and "pfinalize" (the
+ * iteration's three dot-product finalizes) of the precision form. */
 int isph_prof_enable(isph_ctx *ctx, int32_t on);
 int isph_prof_read(isph_ctx *ctx, const char *kernel, double *total_ms,
                    int64_t *launches);
