@@ -49,7 +49,9 @@ Three more legs run by default, at bounded sizes, and go into the same line:
   batched  64 independent porous-beach flumes, N ~ U[20k, 300k] (about 10M
            particles), precision form, stepped together (one launch per
            particle phase and one set of solver launches for all), 20 steps,
-           against the same scenarios stepped one after another alone;
+           against the same scenarios stepped one after another alone; then
+           20 more batched steps with the phase sequences replayed as HIP
+           graphs (graph_*);
   served   a seeded Poisson stream (2 requests/s) of 8 porous-beach flumes
            with N log-uniform on [10k, 1M], 2,000 steps each, admitted
            against a 240 GB HBM budget and continuously batched; p50/p99
@@ -175,6 +177,14 @@ def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int, tune: bool) -> d
     wall = time.perf_counter() - t0
     its = [b.stats(q).iterations for q in range(len(scen))]
     launches = b.launches()
+    # the same batch's next steps with the fixed-shape phase sequences
+    # replayed as HIP graphs (captured in one untimed step)
+    b.set_graph(True)
+    b.step(1)
+    t0 = time.perf_counter()
+    b.step(steps)
+    wall_graph = time.perf_counter() - t0
+    its_graph = [b.stats(q).iterations for q in range(len(scen))]
     b.close()
     n_live = [sc.n for sc in scen]
     for sc in scen:
@@ -200,7 +210,10 @@ def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int, tune: bool) -> d
                 particle_phase_launches_per_step=launches, launch_shapes=shapes,
                 solids=int(sum(c.ns for c in mix)),
                 iterations_last_step_mean=float(np.mean(its)),
-                iterations_last_step_max=int(max(its)))
+                iterations_last_step_max=int(max(its)),
+                graph_seconds=wall_graph,
+                graph_particle_steps_per_s=sum(n_live) * steps / wall_graph,
+                graph_iterations_last_step_mean=float(np.mean(its_graph)))
 
 
 def run_serve(pkg, cases, n_req: int, steps: int, rate: float) -> dict:
