@@ -63,6 +63,10 @@ outside every timed region, and results do not depend on the shape.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode both|plain|chosen]
                        [--no-batch] [--no-serve] [--no-cpu] [--no-tune]
+       python bench.py --flume-full 2000   (the 650k flume for its configured
+                                             2,000 steps: FP64 at 1e-10, FP64 and the
+                                             precision form at --tol, compared at the
+                                             end; not part of the default run)
        python bench.py --copy-calibration   (a 1 GiB device copy, for the
                                              counter calibration only)
 """
@@ -329,6 +333,72 @@ def run_runup_errors(pkg, cases, steps: int) -> dict:
     return out
 
 
+def run_flume_full(pkg, cases, steps: int, tol: float) -> dict:
+    """The 650k porous-beach flume for its configured step count, from the
+    same initial state, three times: the FP64 form at 1e-10 (the reference
+    run), the FP64 form at `tol` and the precision form at `tol`.  Reports
+    whether every solve met its tolerance, each run's wall time, and at the
+    end each `tol` run's differences from the reference run (probe pressures
+    along the flume, paired particles, free-surface elevation at stations):
+    the FP64 run at `tol` shows how far a solve tolerance of `tol` alone
+    carries a run over these steps, the precision run adds its own form."""
+    from beach_slope_amd import gauges
+    c = cases.flume_650k()
+    m = c.meta
+    h = c.params["h"]
+    out = {"workload": f"flume_650k n={c.n} ns={c.ns}", "steps": steps, "tol": tol}
+    states = {}
+    for form, prec, tl in (("fp64_ref", 0, 1e-10), ("fp64", 0, tol), ("precision", 1, tol)):
+        prm = dict(c.params, precision=prec, tol=tl, max_outer=20)
+        sc = pkg.Scenario(c, params=prm)
+        sc.init_state()
+        its, worst, unconv = [], 0.0, 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            _, st = sc.step(1)
+            its.append(st.iterations)
+            worst = max(worst, st.rel_residual)
+            unconv += 0 if st.converged else 1
+            if (k + 1) % 250 == 0:
+                print(f"bench: flume {form} step {k + 1}: {st.iterations} iterations",
+                      file=sys.stderr, flush=True)
+        out[f"{form}_seconds"] = time.perf_counter() - t0
+        out[f"{form}_iterations_mean"] = float(np.mean(its))
+        out[f"{form}_worst_rel_residual"] = worst
+        out[f"{form}_unconverged_solves"] = unconv
+        out[f"{form}_t_end_s"] = sc.scalars().t
+        states[form] = sc.download()
+        sc.close()
+    ref = states["fp64_ref"]
+    off, offy = m["off"], m["offy"]
+    probes = np.array([[off + x, offy + y] for x, y in
+                       ((0.5, 0.2), (2.0, 0.2), (4.0, 0.2), (5.5, 0.2), (7.0, 0.25))])
+    stations = off + np.array([0.3, 1.0, 2.0, 4.0, 5.5, 6.5, 7.5])
+    p_ref = gauges.probe_pressure(ref, probes, h)
+    e_ref = gauges.surface_elevation(ref, stations, 0.5 * m["dx"])
+    fl_r = gauges.fluid(ref)
+    out.update(speedup_precision_vs_fp64=out["fp64_seconds"] / out["precision_seconds"],
+               probe_pressure_pa=[float(v) for v in p_ref], surface_m=[float(v) for v in e_ref],
+               max_speed_m_s=float(np.hypot(ref["u"][fl_r], ref["v"][fl_r]).max()))
+    for form in ("fp64", "precision"):
+        a = states[form]
+        pair = gauges.pair_states(a, ref)
+        fl_a = gauges.fluid(a)
+        pa = gauges.probe_pressure(a, probes, h)
+        ea = gauges.surface_elevation(a, stations, 0.5 * m["dx"])
+        out[f"{form}_vs_ref"] = dict(
+            probe_pressure_rel_l2=float(np.linalg.norm(pa - p_ref) / np.linalg.norm(p_ref)),
+            surface_max_abs_m=float(np.nanmax(np.abs(ea - e_ref))),
+            paired_pressure_rel_l2=pair["pressure_rel_l2"],
+            paired_velocity_rel_l2=pair["velocity_rel_l2"],
+            paired_max_distance_m=pair["max_pair_distance"],
+            free_flag_mismatch=pair["free_flag_mismatch"],
+            particles=int(a["x"].size),
+            fluid_volume_m2=float(np.sum(a["m"][fl_a] / a["rho"][fl_a])))
+    out["ref_fluid_volume_m2"] = float(np.sum(ref["m"][fl_r] / ref["rho"][fl_r]))
+    return out
+
+
 KERNEL_OF = {"spmv_p": "k_bicg_1", "spmv_s": "k_bicg_2", "spmv_p32": "kb_pb_1",
              "spmv_s32": "kb_pb_2", "mg_pre0": "kb_mg_pre0", "mg_post0": "kb_mg_post0",
              "mg_coarse": "kb_mg_coarse", "mg_restrict0": "kb_mg_restrict0",
@@ -417,6 +487,10 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=8)
     ap.add_argument("--serve-steps", type=int, default=2000)
+    ap.add_argument("--flume-full", type=int, default=0, metavar="STEPS",
+                    help="only run the 650k flume for STEPS steps (FP64 at 1e-10, FP64 and "
+                         "the precision form at --tol) and print that comparison (the "
+                         "configuration's 2,000 steps take ~7 min)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -427,6 +501,9 @@ def main() -> int:
         return 2
     pkg = entry.load_package()
     from beach_slope_amd import case as cases
+    if args.flume_full > 0:
+        print(json.dumps(run_flume_full(pkg, cases, args.flume_full, args.tol)), flush=True)
+        return 0
     c = cases.flume_650k()
     print(f"bench: workload flume_650k n={c.n} steps={args.steps} warmup={args.warmup}",
           file=sys.stderr, flush=True)
