@@ -63,6 +63,9 @@ outside every timed region, and results do not depend on the shape.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode both|plain|chosen]
                        [--no-batch] [--no-serve] [--no-cpu] [--no-tune]
+       python bench.py --batch-full 1000   (the 64 batched flumes for their configured
+                                             1,000 steps, every solve checked; not
+                                             part of the default run)
        python bench.py --flume-full 2000   (the 650k flume for its configured
                                              2,000 steps: FP64 at 1e-10, FP64 and the
                                              precision form at --tol, compared at the
@@ -218,6 +221,46 @@ def run_batch(pkg, cases, n_scen: int, steps: int, warmup: int, tune: bool) -> d
                 graph_seconds=wall_graph,
                 graph_particle_steps_per_s=sum(n_live) * steps / wall_graph,
                 graph_iterations_last_step_mean=float(np.mean(its_graph)))
+
+
+def run_batch_full(pkg, cases, n_scen: int, steps: int, tune: bool) -> dict:
+    """The batched configuration for its configured step count: the 64 flumes
+    of the batched leg stepped together `steps` steps after one warm-up step
+    (and the launch-shape tuning), with every scenario's solve checked at
+    every step; scenarios/h is measured, not extrapolated."""
+    mix = cases.scenario_mix(n_scen=n_scen, seed=2024, n_min=20_000, n_max=300_000)
+    for c in mix:
+        c.params.update(precision=1, tol=1e-8, max_outer=20)
+    scen = [pkg.Scenario(c) for c in mix]
+    for sc in scen:
+        sc.init_state()
+    b = pkg.Batch(scen)
+    b.step(1)
+    shapes = _shapes(b.autotune(trials=2)) if tune else None
+    n0 = [b.count(q) for q in range(n_scen)]
+    its_sum, its_max, worst, unconv = 0, 0, 0.0, 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        b.step(1)
+        for q in range(n_scen):
+            st = b.stats(q)
+            its_sum += st.iterations
+            its_max = max(its_max, st.iterations)
+            worst = max(worst, st.rel_residual)
+            unconv += 0 if st.converged else 1
+        if (k + 1) % 100 == 0:
+            print(f"bench: batch step {k + 1}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    n1 = [b.count(q) for q in range(n_scen)]
+    b.close()
+    for sc in scen:
+        sc.close()
+    return dict(scenarios=n_scen, particles=int(sum(n0)), solids=int(sum(c.ns for c in mix)),
+                steps=steps, seconds=wall, particle_steps_per_s=sum(n0) * steps / wall,
+                scenarios_per_h=3600.0 * n_scen / wall, launch_shapes=shapes,
+                iterations_mean=its_sum / (steps * n_scen), iterations_max=its_max,
+                worst_rel_residual=worst, unconverged_solves=unconv,
+                particles_kept=n0 == n1)
 
 
 def run_serve(pkg, cases, n_req: int, steps: int, rate: float) -> dict:
@@ -487,6 +530,9 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=8)
     ap.add_argument("--serve-steps", type=int, default=2000)
+    ap.add_argument("--batch-full", type=int, default=0, metavar="STEPS",
+                    help="only run the batched configuration (64 flumes) for STEPS steps, "
+                         "every solve checked, and print that line (1,000 steps: ~2.5 min)")
     ap.add_argument("--flume-full", type=int, default=0, metavar="STEPS",
                     help="only run the 650k flume for STEPS steps (FP64 at 1e-10, FP64 and "
                          "the precision form at --tol) and print that comparison (the "
@@ -503,6 +549,10 @@ def main() -> int:
     from beach_slope_amd import case as cases
     if args.flume_full > 0:
         print(json.dumps(run_flume_full(pkg, cases, args.flume_full, args.tol)), flush=True)
+        return 0
+    if args.batch_full > 0:
+        print(json.dumps(run_batch_full(pkg, cases, args.batch_scenarios, args.batch_full,
+                                        not args.no_tune)), flush=True)
         return 0
     c = cases.flume_650k()
     print(f"bench: workload flume_650k n={c.n} steps={args.steps} warmup={args.warmup}",
