@@ -73,10 +73,6 @@ __global__ void __launch_bounds__(256) kb_mg_setup0(BatchDev B) {
   BATCH_ROW(B);
   mg_setup0_row(D, B.mgs[s_], vb_ * RED_BLOCK + threadIdx.x);
 }
-__global__ void __launch_bounds__(256) kb_mg_rowst(BatchDev B) {
-  BATCH_ROW(B);
-  mg_rowst_row(D, B.mgs[s_], vb_ * RED_BLOCK + threadIdx.x);
-}
 __global__ void __launch_bounds__(256) kb_mg_stencil1(BatchDev B) {
   BATCH_CELLS(B, BK_CELL1);
   mg_stencil1_cell(D, G, I_);

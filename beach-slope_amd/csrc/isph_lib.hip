@@ -1058,7 +1058,6 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   const int grow = 8 * ((tot(BK_ROWS) + 7) / 8);
   // multigrid hierarchy from the assembled rows
   BLAUNCH(kb_mg_setup0, grow, 256, B);
-  BLAUNCH(kb_mg_rowst, grow, 256, B);
   BLAUNCH(kb_mg_stencil1, tot(BK_CELL1), 256, B);
   for (int l = 1; l < b->max_nlev; l++)
     if (tot(BK_COARSEN + l - 1) > 0) BLAUNCH(kb_mg_coarsen, tot(BK_COARSEN + l - 1), 256, B, l);

@@ -153,57 +153,68 @@ __device__ __forceinline__ unsigned short mg_half_bits(float v) {
   return __half_as_ushort(__float2half_rn(cl));
 }
 
-// level 0, row i: inverse row diagonal (diagonal plus the slot-1 value of
-// ghost rows), the row's FP16 values for the V-cycle, and its packed 16-bit
-// column offsets (a row with an
-// offset beyond 16 bits raises the scenario's far flag, and the solve reads
-// the int32 columns)
+// level 0 and the level-1 set-up, row i, one pass over the row's entries:
+//  * the inverse row diagonal (diagonal plus the slot-1 value of ghost
+//    rows), the row's FP16 values for the V-cycle, and its packed 16-bit
+//    column offsets (a row with an offset beyond 16 bits raises the
+//    scenario's far flag, and the solve reads the int32 columns);
+//  * the row's entries summed into the nine neighbour-cell bins of its own
+//    level-1 cell, in entry order (an entry whose particle lags outside the
+//    3 x 3 cells is left out).
+// The entries are read eight at a time (columns and values, then the
+// columns' cells) before any is used; the sums run in entry order.
+#define MG_SETUP_U 8
 __device__ __forceinline__ void mg_setup0_row(const Dev& D, const MgDev& G, int i) {
   if (i >= D.sc->n) return;
-  const int cap = D.cap, L = D.alen[i];
-  double d = 0.0;
-  bool far = false;
-  for (int k = 0; k < L; k += 2) {
-    const int j0 = D.acol[k * cap + i];
-    const int j1 = (k + 1 < L) ? D.acol[(k + 1) * cap + i] : i;
-    if (j0 == i) d = d + D.aval[k * cap + i];
-    if (k + 1 < L && j1 == i) d = d + D.aval[(k + 1) * cap + i];
-    const int o0 = j0 - i, o1 = j1 - i;
-    far = far || o0 < -32768 || o0 > 32767 || o1 < -32768 || o1 > 32767;
-    D.aoff[(k >> 1) * cap + i] = (uint32_t)(uint16_t)(int16_t)o0 |
-                                 ((uint32_t)(uint16_t)(int16_t)o1 << 16);
-    const float v0 = (float)D.aval[k * cap + i];
-    const float v1 = (k + 1 < L) ? (float)D.aval[(k + 1) * cap + i] : 0.0f;
-    D.av16[(k >> 1) * cap + i] = (uint32_t)mg_half_bits(v0) | ((uint32_t)mg_half_bits(v1) << 16);
-  }
-  if (far) atomicOr(&D.sc->far, 1);
-  G.d0[i] = (d != 0.0 && isfinite(d)) ? (float)(1.0 / d) : 1.0f;
-}
-
-// level 1, row pass: row i's entries summed into the nine neighbour-cell bins
-// of its own cell, in entry order
-__device__ __forceinline__ void mg_rowst_row(const Dev& D, const MgDev& G, int i) {
-  if (i >= D.sc->n) return;
   const MgLevel L = G.lev[1];
-  double s[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) s[k] = 0.0;
   const int cap = D.cap, Lr = D.alen[i];
   const int I = D.cell[i];
   const int cjI = I / L.ny, ciI = I - cjI * L.ny;
-  for (int k = 0; k < Lr; k++) {
-    const int j = D.acol[k * cap + i];
-    const int J = D.cell[j];
-    const int cjJ = J / L.ny, ciJ = J - cjJ * L.ny;
-    const int di = ciJ - ciI, dj = cjJ - cjI;
-    if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;  // a lagging particle
-    const double a = D.aval[k * cap + i];
+  double d = 0.0, st[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++)
-      if (q == (dj + 1) * 3 + (di + 1)) s[q] = s[q] + a;
+  for (int q = 0; q < 9; q++) st[q] = 0.0;
+  bool far = false;
+  static_assert(MG_SETUP_U % 2 == 0, "entries are packed two to a word");
+  for (int k0 = 0; k0 < Lr; k0 += MG_SETUP_U) {
+    int j[MG_SETUP_U], J[MG_SETUP_U];
+    double a[MG_SETUP_U];
+#pragma unroll
+    for (int u = 0; u < MG_SETUP_U; u++) {
+      const bool in = k0 + u < Lr;
+      j[u] = in ? D.acol[(k0 + u) * cap + i] : i;
+      a[u] = in ? D.aval[(k0 + u) * cap + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < MG_SETUP_U; u++) J[u] = D.cell[j[u]];
+#pragma unroll
+    for (int u = 0; u < MG_SETUP_U; u++) {
+      if (k0 + u >= Lr) break;
+      if (j[u] == i) d = d + a[u];
+      const int cjJ = J[u] / L.ny, ciJ = J[u] - cjJ * L.ny;
+      const int di = ciJ - ciI, dj = cjJ - cjI;
+      if (di < -1 || di > 1 || dj < -1 || dj > 1) continue;  // a lagging particle
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        if (q == (dj + 1) * 3 + (di + 1)) st[q] = st[q] + a[u];
+    }
+#pragma unroll
+    for (int u = 0; u < MG_SETUP_U; u += 2) {
+      const int k = k0 + u;
+      if (k >= Lr) break;
+      const int o0 = j[u] - i, o1 = j[u + 1] - i;  // j = i past the row's end: offset 0
+      far = far || o0 < -32768 || o0 > 32767 || o1 < -32768 || o1 > 32767;
+      D.aoff[(k >> 1) * cap + i] = (uint32_t)(uint16_t)(int16_t)o0 |
+                                   ((uint32_t)(uint16_t)(int16_t)o1 << 16);
+      const float v0 = (float)a[u];
+      const float v1 = (k + 1 < Lr) ? (float)a[u + 1] : 0.0f;
+      D.av16[(k >> 1) * cap + i] =
+          (uint32_t)mg_half_bits(v0) | ((uint32_t)mg_half_bits(v1) << 16);
+    }
   }
+  if (far) atomicOr(&D.sc->far, 1);
+  G.d0[i] = (d != 0.0 && isfinite(d)) ? (float)(1.0 / d) : 1.0f;
 #pragma unroll
-  for (int q = 0; q < 9; q++) G.rowst[q * cap + i] = s[q];
+  for (int q = 0; q < 9; q++) G.rowst[q * cap + i] = st[q];
 }
 
 // level 1, cell pass: the row sums of cell I's contiguous row range, in row
