@@ -963,13 +963,29 @@ __device__ __forceinline__ void cell_max_cell(const Dev& D, int k, int ncell, in
   if (k < ncell) {
     const int i0 = D.cstart[k], i1 = D.cstart[k + 1];
     if (i1 > i0) {
-      for (int i = i0; i < i1; i++) {
-        if (!init && D.tid[i] == TID_FLUID && !(D.flags[i] & F_BUFFER)) {
-          const unsigned long long b = max_bits(D.vel[i]);
-          vb = (b > vb) ? b : vb;
+      // eight rows' fields requested before any is used (a maximum of
+      // non-negative bit patterns does not depend on the order)
+      for (int b0 = i0; b0 < i1; b0 += 8) {
+        uint8_t tb[8], fb[8];
+        double vl[8], nt[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = (b0 + u < i1) ? b0 + u : i0;
+          tb[u] = D.tid[i];
+          fb[u] = D.flags[i];
+          vl[u] = D.vel[i];
+          nt[u] = D.nut[i];
         }
-        const unsigned long long b = max_bits(D.nut[i]);
-        nb = (b > nb) ? b : nb;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (b0 + u >= i1) break;
+          if (!init && tb[u] == TID_FLUID && !(fb[u] & F_BUFFER)) {
+            const unsigned long long b = max_bits(vl[u]);
+            vb = (b > vb) ? b : vb;
+          }
+          const unsigned long long b = max_bits(nt[u]);
+          nb = (b > nb) ? b : nb;
+        }
       }
       D.cmaxv[k] = vb;
       D.cmaxn[k] = nb;
