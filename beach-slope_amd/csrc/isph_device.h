@@ -248,21 +248,36 @@ __device__ __forceinline__ double d_wab(double r, const DevParams& P) {
   return ((0.1 * t1) * P.k1) + (((1.0 - 0.1) * t2) * P.k2);
 }
 
-// Wabx / Waby (kernel.f90:94-162) and hWabx / hWaby (kernel.f90:204-248) for
-// the pair (p1 = j, p2 = i): d_ij = coordinate of i minus coordinate of j.
-__device__ __forceinline__ double d_wabd(double d_ij, double r, const DevParams& P) {
+// Wabx and Waby (kernel.f90:94-162; hWabx / hWaby, kernel.f90:204-248, are
+// identical) of one pair (p1 = j, p2 = i), with (dx, dy) = coordinates of i
+// minus those of j.  Each component is the reference's expression
+//   r < h:  t1 = ((9 r / h^3) - 12 / h^2) d,  t2 = ((3 r / h^3) - 6 / (r h)) d
+//   else:   t1 = t2 = ((3 / (r h)) (2 - r / h)^2) (-d)
+//   W'     = (0.1 t1) k1 + (0.9 t2) k2,
+// rounded as written; the bracketed factors depend on r alone, so they are
+// formed once (their divisions are most of the pair's FP64 work) and
+// multiply both components, which gives each component the same bits.
+__device__ __forceinline__ void d_wabd2(double dx, double dy, double r, const DevParams& P,
+                                        double& kx, double& ky) {
   const double h = P.h;
-  double t1, t2;
+  double t1x, t2x, t1y, t2y;
   if (r < h) {
-    t1 = (((9.0 * r) / ((h * h) * h)) - (12.0 / (h * h))) * d_ij;
-    t2 = (((3.0 * r) / ((h * h) * h)) - (6.0 / (r * h))) * d_ij;
+    const double f1 = ((9.0 * r) / ((h * h) * h)) - (12.0 / (h * h));
+    const double f2 = ((3.0 * r) / ((h * h) * h)) - (6.0 / (r * h));
+    t1x = f1 * dx;
+    t2x = f2 * dx;
+    t1y = f1 * dy;
+    t2y = f2 * dy;
   } else {
     const double a = 2.0 - r / h;
     const double f = (3.0 / (r * h)) * (a * a);
-    t1 = f * (-d_ij);
-    t2 = t1;
+    t1x = f * (-dx);
+    t2x = t1x;
+    t1y = f * (-dy);
+    t2y = t1y;
   }
-  return ((0.1 * t1) * P.k1) + (((1.0 - 0.1) * t2) * P.k2);
+  kx = ((0.1 * t1x) * P.k1) + (((1.0 - 0.1) * t2x) * P.k2);
+  ky = ((0.1 * t1y) * P.k1) + (((1.0 - 0.1) * t2y) * P.k2);
 }
 
 // kernel value and gradient pair for one neighbour (both components share r)
@@ -272,8 +287,7 @@ struct Klt {
 __device__ __forceinline__ Klt d_klt(double dx, double dy, const DevParams& P) {
   Klt o;
   o.r = sqrt((dx * dx) + (dy * dy));  // dist, functions.f90:28-42
-  o.k1 = d_wabd(dx, o.r, P);
-  o.k2 = d_wabd(dy, o.r, P);
+  d_wabd2(dx, dy, o.r, P, o.k1, o.k2);
   return o;
 }
 

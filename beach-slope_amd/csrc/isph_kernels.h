@@ -354,7 +354,8 @@ __device__ __forceinline__ void neighbour_core(const Dev& D, const int i, POS po
       const double xj = xb[q], yj = yb[q];
       const double dx = xi - xj, dy = yi - yj;
       const double r = sqrt((dx * dx) + (dy * dy));
-      const double k1 = d_wabd(dx, r, P), k2 = d_wabd(dy, r, P);
+      double k1, k2;
+      d_wabd2(dx, dy, r, P, k1, k2);
       const double mj = mb[q], rj = rb[q];
       const double r1 = (mj * (xj - xi)) / rj;
       const double r3 = (mj * (yj - yi)) / rj;
@@ -1258,7 +1259,8 @@ __device__ __forceinline__ void scalart_body(const Dev& D, const int i, const in
       else t1x = ((2 * Dxxi) * Dxxj) / (Dxxi + Dxxj);
     }
     const double t3 = (vol0i * (ni + nj)) / (ni * nj);
-    const double hwx = d_wabd(dx, r, P), hwy = d_wabd(dy, r, P);
+    double hwx, hwy;
+    d_wabd2(dx, dy, r, P, hwx, hwy);
     const double den = (r * r) + P.lam;
     cd = (cd + (((((((t1x * 1.0) * t3) * (ci1 - c2)) * dx) * hwx) / den))) +
          (((((((t1x * 1.0) * t3) * (ci1 - c2)) * dy) * hwy) / den));
