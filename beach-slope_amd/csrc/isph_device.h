@@ -256,21 +256,28 @@ __device__ __forceinline__ double d_wab(double r, const DevParams& P) {
 //   W'     = (0.1 t1) k1 + (0.9 t2) k2,
 // rounded as written; the bracketed factors depend on r alone, so they are
 // formed once (their divisions are most of the pair's FP64 work) and
-// multiply both components, which gives each component the same bits.
+// multiply both components, which gives each component the same bits.  The
+// lanes of a wavefront take both branches, so the divisions of the two
+// branches share instructions: lane by lane the first divides 9r by h^3 or
+// r by h, the second 3r by h^3 or 3 by r h, each its own branch's operands.
 __device__ __forceinline__ void d_wabd2(double dx, double dy, double r, const DevParams& P,
                                         double& kx, double& ky) {
   const double h = P.h;
+  const bool nr = r < h;
+  const double hhh = (h * h) * h, rh = r * h;
+  const double qa = (nr ? 9.0 * r : r) / (nr ? hhh : h);      // 9r / h^3  |  r / h
+  const double qb = (nr ? 3.0 * r : 3.0) / (nr ? hhh : rh);   // 3r / h^3  |  3 / (r h)
   double t1x, t2x, t1y, t2y;
-  if (r < h) {
-    const double f1 = ((9.0 * r) / ((h * h) * h)) - (12.0 / (h * h));
-    const double f2 = ((3.0 * r) / ((h * h) * h)) - (6.0 / (r * h));
+  if (nr) {
+    const double f1 = qa - (12.0 / (h * h));
+    const double f2 = qb - (6.0 / rh);
     t1x = f1 * dx;
     t2x = f2 * dx;
     t1y = f1 * dy;
     t2y = f2 * dy;
   } else {
-    const double a = 2.0 - r / h;
-    const double f = (3.0 / (r * h)) * (a * a);
+    const double a = 2.0 - qa;
+    const double f = qb * (a * a);
     t1x = f * (-dx);
     t2x = t1x;
     t1y = f * (-dy);
