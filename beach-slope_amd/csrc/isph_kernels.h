@@ -609,13 +609,28 @@ __device__ __forceinline__ void int_vel_varts_body(const Dev& D, const int i, co
   const double xi = D.xnb[i], yi = D.ynb[i];
   const double ui = D.u[i] / D.por[i], vi = D.v[i] / D.por[i];
   double t1 = 0.0, t2 = 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = D.nbr[k * cap + i];
-    const Klt z = d_klt(xi - D.xnb[j], yi - D.ynb[j], P);
-    const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
-    const double nj = D.por[j], vj = D.m[j] / D.rho[j];
-    t1 = t1 + (((D.u[j] / nj) - ui) * gx) * vj;
-    t2 = t2 + (((D.v[j] / nj) - vi) * gy) * vj;
+  // neighbours two at a time: both neighbours' fields are requested before
+  // either is used (the sums stay in list order)
+  for (int k0 = 0; k0 < cnt; k0 += 2) {
+    int jb[2];
+    double xb[2], yb[2], nb[2], mb[2], rb[2], ub[2], vb[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) jb[q] = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int j = jb[q];
+      xb[q] = D.xnb[j]; yb[q] = D.ynb[j]; nb[q] = D.por[j]; mb[q] = D.m[j];
+      rb[q] = D.rho[j]; ub[q] = D.u[j]; vb[q] = D.v[j];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (k0 + q >= cnt) break;
+      const Klt z = d_klt(xi - xb[q], yi - yb[q], P);
+      const double gx = c1 * z.k1 + c2 * z.k2, gy = c3 * z.k1 + c4 * z.k2;
+      const double nj = nb[q], vj = mb[q] / rb[q];
+      t1 = t1 + (((ub[q] / nj) - ui) * gx) * vj;
+      t2 = t2 + (((vb[q] / nj) - vi) * gy) * vj;
+    }
   }
   const double uu = D.u[i], vv = D.v[i];
   D.varts[i] = (((1.0 * (P.dl * P.dl)) * sqrt((t1 * t1) + (t2 * t2))) * sqrt((uu * uu) + (vv * vv))) /
@@ -650,15 +665,17 @@ __device__ __forceinline__ void int_vel_body(const Dev& D, const int i, const in
     for (int k0 = 0; k0 < cnt; k0 += 2) {
     int jb[2], tb[2];
     uint8_t fb[2];
-    double xnb_[2], ynb_[2], mb[2], rb[2], xb[2], yb[2], ub[2], vb[2], kb[2];
+    double xnb_[2], ynb_[2], mb[2], rb[2], xb[2], yb[2], ub[2], vb[2], kb[2], r0b[2], ntb[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) jb[q] = (k0 + q < cnt) ? D.nbr[(k0 + q) * cap + i] : i;
+    // rho0 and nu_t of the neighbour are used by fluid neighbours only (most
+    // of them): read with the other fields rather than after the type test
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int j = jb[q];
       xnb_[q] = D.xnb[j]; ynb_[q] = D.ynb[j]; tb[q] = D.tid[j]; fb[q] = D.flags[j];
       mb[q] = D.m[j]; rb[q] = D.rho[j]; xb[q] = D.x[j]; yb[q] = D.y[j];
-      ub[q] = D.u[j]; vb[q] = D.v[j]; kb[q] = D.tke[j];
+      ub[q] = D.u[j]; vb[q] = D.v[j]; kb[q] = D.tke[j]; r0b[q] = D.rho0[j]; ntb[q] = D.nut[j];
     }
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -669,7 +686,7 @@ __device__ __forceinline__ void int_vel_body(const Dev& D, const int i, const in
       const int tj = tb[q];
       double cmu;
       if (tj == TID_FLUID) {
-        const double r0j = D.rho0[j], visj = d_vis(r0j);
+        const double r0j = r0b[q], visj = d_vis(r0j);
         cmu = (((2 * visi) * visj) * ((1.0 / r0i) * (1.0 / r0j))) / ((visi / r0i) + (visj / r0j));
       } else {
         cmu = cmu_wall;
@@ -685,18 +702,25 @@ __device__ __forceinline__ void int_vel_body(const Dev& D, const int i, const in
       double a1 = ((((2 * pvol) * (ui - uj)) * dxij) * gx) * pd;
       double a2 = ((((2 * pvol) * (ui - uj)) * dyij) * gy) * pd;
       us = us + (a1 + a2) * (cmu + vart);
-      double nutj = 0.0;
+      // the SPS viscosity of the pair (the harmonic mean is the same value
+      // in both components, so it is divided once)
+      double nutj = 0.0, hm = 0.0;
+      bool low = false;
       if (tj > 2) {
-        nutj = D.nut[j];
-        if ((nuti < eps6) && (nutj < eps6)) us = us + ((a1 + a2) * (nuti + nutj)) * 0.50;
-        else us = us + (a1 + a2) * (((2 * nuti) * nutj) / (nuti + nutj));
+        nutj = ntb[q];
+        low = (nuti < eps6) && (nutj < eps6);
+        if (!low) hm = ((2 * nuti) * nutj) / (nuti + nutj);
+      }
+      if (tj > 2) {
+        if (low) us = us + ((a1 + a2) * (nuti + nutj)) * 0.50;
+        else us = us + (a1 + a2) * hm;
       }
       a1 = ((((2 * pvol) * (vi - vj)) * dxij) * gx) * pd;
       a2 = ((((2 * pvol) * (vi - vj)) * dyij) * gy) * pd;
       vs = vs + (a1 + a2) * (cmu + vart);
       if (tj > 2) {
-        if ((nuti < eps6) && (nutj < eps6)) vs = vs + ((a1 + a2) * (nuti + nutj)) * 0.50;
-        else vs = vs + (a1 + a2) * (((2 * nuti) * nutj) / (nuti + nutj));
+        if (low) vs = vs + ((a1 + a2) * (nuti + nutj)) * 0.50;
+        else vs = vs + (a1 + a2) * hm;
       }
       const double dk = pvol * (kb[q] - tkei);
       double tk = dk * gx;
