@@ -503,14 +503,15 @@ __device__ __forceinline__ void effpor_body(const Dev& D, const int i, const int
   else { di[1] = -1; di[2] = -1; dj[2] = -1; dj[3] = -1; }
   const double por = P.por;
   double sf = 0.0;
-  // eight solids' positions requested before any is tested (the sum stays in
-  // solid order)
+  // eight solids' positions requested before any is tested, then the mass
+  // and density of those within reach requested before any is used (the sum
+  // stays in solid order)
   constexpr int NB_SOL = 8;
   for (int cb = 0; cb < 4; cb++) {
     const int k = (cj + dj[cb]) * P.celly + (ci + di[cb]);
     const int qe = D.sstart[k + 1];
     for (int s0 = D.sstart[k]; s0 < qe; s0 += NB_SOL) {
-      double sx[NB_SOL], sy[NB_SOL];
+      double sx[NB_SOL], sy[NB_SOL], d2[NB_SOL], sm[NB_SOL], sr[NB_SOL];
 #pragma unroll
       for (int u = 0; u < NB_SOL; u++) {
         const int s = (s0 + u < qe) ? s0 + u : s0;
@@ -519,13 +520,20 @@ __device__ __forceinline__ void effpor_body(const Dev& D, const int i, const int
       }
 #pragma unroll
       for (int u = 0; u < NB_SOL; u++) {
-        if (s0 + u >= qe) break;
         const double dx = x - sx[u], dy = y - sy[u];
-        const double d2 = (dx * dx) + (dy * dy);
-        if (d2 > P.lim2) continue;  // certainly beyond 2h: no square root
-        const double r = sqrt(d2);
-        if (r < P.two_h)
-          sf = sf + ((((1.0 - por * 1.0) * d_wab(r, P)) * D.sm[s0 + u]) / D.srho[s0 + u]);
+        d2[u] = (dx * dx) + (dy * dy);
+        // certainly beyond 2h (or past the cell's solids): nothing to read
+        const bool in = (s0 + u < qe) && !(d2[u] > P.lim2);
+        const int s = in ? s0 + u : s0;
+        sm[u] = in ? D.sm[s] : 0.0;
+        sr[u] = in ? D.srho[s] : 1.0;
+      }
+#pragma unroll
+      for (int u = 0; u < NB_SOL; u++) {
+        if (s0 + u >= qe) break;
+        if (d2[u] > P.lim2) continue;  // no square root
+        const double r = sqrt(d2[u]);
+        if (r < P.two_h) sf = sf + ((((1.0 - por * 1.0) * d_wab(r, P)) * sm[u]) / sr[u]);
       }
     }
   }
