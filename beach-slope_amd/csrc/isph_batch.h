@@ -133,18 +133,13 @@ __global__ void __launch_bounds__(256) kb_pb_s(BatchDev B) {
 }
 template <bool OFF>
 __global__ void __launch_bounds__(256) kb_pb_2(BatchDev B) {
-  __shared__ double lds[RED_BLOCK];
+  __shared__ double lds[5 * RED_BLOCK];
   BATCH_ROW(B);
   pb_2_rows<OFF>(D, vb_, lds);
 }
-__global__ void __launch_bounds__(256) kb_pb_3(BatchDev B) {
-  __shared__ double lds[RED_BLOCK];
+__global__ void __launch_bounds__(256) kb_pb_34(BatchDev B, int seq) {
   BATCH_ROW(B);
-  pb_3_rows(D, vb_, lds);
-}
-__global__ void __launch_bounds__(256) kb_pb_4(BatchDev B) {
-  BATCH_ROW(B);
-  pb_4_rows(D, B.mgs[s_], vb_);
+  pb_34_rows(D, B.mgs[s_], vb_, seq);
 }
 __global__ void __launch_bounds__(256) kb_pb_resid(BatchDev B, int init) {
   __shared__ double lds[RED_BLOCK];
@@ -165,9 +160,9 @@ __global__ void __launch_bounds__(256) kb_copy_sol_to_p(BatchDev B) {
 }
 
 // one workgroup per scenario
-__global__ void __launch_bounds__(256) kb_pfinalize(BatchDev B, int op, int init) {
-  __shared__ double lds[RED_BLOCK];
-  pb_finalize(B.devs[blockIdx.x], op, init, lds);
+__global__ void __launch_bounds__(256) kb_pfinalize(BatchDev B, int op, int init, int seq) {
+  __shared__ double lds[5 * RED_BLOCK];
+  pb_finalize(B.devs[blockIdx.x], op, init, seq, lds);
 }
 
 // copy every scenario's scalars into one array (one host read per poll)

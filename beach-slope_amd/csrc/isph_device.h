@@ -45,7 +45,9 @@ struct DevParams {
 struct DevScalars {
   double t, dt, dtsol, umax, numax;
   unsigned long long umax_bits, numax_bits;  // unused (layout kept)
-  int solsteps, n, n_hi_unused, err;
+  int solsteps, n;
+  int stop_seq;  // precision form: the iteration (launch sequence number) that stopped
+  int err;
   // BiCGStab state
   double rho, alpha, omega, beta, rr, bnorm, stop, rel;
   int iters, done, conv, brk;
@@ -347,6 +349,32 @@ __device__ __forceinline__ double block_tree(double val, double* lds) {
   return v;  // valid in thread 0
 }
 
+// block_tree of N values at once, each summed in block_tree's order (results
+// in thread 0's v); lds holds N * RED_BLOCK doubles
+template <int N>
+__device__ __forceinline__ void block_tree_n(double (&v)[N], double* lds) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < N; k++) lds[k * RED_BLOCK + t] = v[k];
+  __syncthreads();
+  if (t < 128) {
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      lds[k * RED_BLOCK + t] = lds[k * RED_BLOCK + t] + lds[k * RED_BLOCK + t + 128];
+  }
+  __syncthreads();
+  if (t < 64) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double a = lds[k * RED_BLOCK + t] + lds[k * RED_BLOCK + t + 64];
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1) a = a + __shfl_down(a, s, 64);
+      v[k] = a;
+    }
+  }
+  __syncthreads();
+}
+
 // maxima over a block of two non-negative double bit patterns (valid in
 // thread 0); every thread of the block must call it
 __device__ __forceinline__ void block_max2_u64(unsigned long long& a, unsigned long long& b) {
@@ -409,4 +437,29 @@ __device__ __forceinline__ void final_tree2(const double* __restrict__ p0,
   }
   s0 = block_tree(a0, lds);
   s1 = p1 ? block_tree(a1, lds) : 0.0;
+}
+
+// final_tree2 over N partial arrays p + k * stride (results in thread 0's s);
+// lds holds N * RED_BLOCK doubles
+template <int N>
+__device__ __forceinline__ void final_tree_n(const double* __restrict__ p, int stride, int nb,
+                                             double* lds, double (&s)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; k++) s[k] = 0.0;
+  for (int k0 = threadIdx.x; k0 < nb; k0 += 8 * RED_BLOCK) {
+    double v[N][8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int j = k0 + u * RED_BLOCK;
+#pragma unroll
+      for (int k = 0; k < N; k++) v[k][u] = (j < nb) ? p[k * stride + j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k0 + u * RED_BLOCK < nb) {
+#pragma unroll
+        for (int k = 0; k < N; k++) s[k] = s[k] + v[k][u];
+      }
+  }
+  block_tree_n<N>(s, lds);
 }

@@ -75,7 +75,7 @@ struct isph_ctx {
   // set by each solve
   KTimer timers[11] = {{"spmv_p", 1},    {"spmv_s", 1},  {"spmv_p32", 1},     {"spmv_s32", 1},
                        {"mg_coarse", 2}, {"mg_pre0", 2}, {"mg_post0", 2},     {"mg_down", 0},
-                       {"mg_up", 0},     {"pfinalize", 3}, {"mg_restrict0", 2}};
+                       {"mg_up", 0},     {"pfinalize", 2}, {"mg_restrict0", 2}};
   Dev D;
   hipStream_t stream = nullptr;
   int64_t cap = 0, scap = 0;
@@ -195,7 +195,7 @@ extern "C" int64_t isph_footprint_bytes(int64_t n_cap, int64_t n_solid_cap, int3
   // vectors, level-0 multigrid vectors, setup rows
   if (precision) per += 2 * 4 * ((ROW_CAP + 1) / 2) + 5 * 4 + 2 * 4 + 9 * 8;
   int64_t b = per * n_cap + 4 * 8 * n_solid_cap + 4 * (ncell * (2 + 2 * NCLS) + 16);
-  b += 2 * 8 * (n_cap / 256 + 2);               // reduction partials
+  b += 5 * 8 * (n_cap / 256 + 2);               // reduction partials
   b += 8 * (2 * ncell + 2 * (ncell / 256 + 2)); // per-cell maxima and their block maxima
   // cell levels: stencil, x, b, r and 1/diagonal per unknown, the levels
   // holding 4/3 of the cell count
@@ -261,7 +261,7 @@ extern "C" int isph_create(const isph_params* params, int64_t n_cap, int64_t n_s
   rc |= dalloc(c, &D.sm, c->scap);
   rc |= dalloc(c, &D.srho, c->scap);
   rc |= dalloc(c, &D.sstart, c->ncell + 1);
-  rc |= dalloc(c, &D.part, 2 * (N / 256 + 2));
+  rc |= dalloc(c, &D.part, 5 * (N / 256 + 2));
   D.nmaxb = (int)(c->ncell / 256 + 2);
   rc |= dalloc(c, &D.maxb, 2 * (int64_t)D.nmaxb);
   rc |= dalloc(c, &D.cmaxv, c->ncell);
@@ -1017,25 +1017,25 @@ static int bh_vcycle(BatchHost* b, const BatchDev& B, int grow) {
   return 0;
 }
 
-// chunk iterations of the preconditioned BiCGStab for every scenario
+// chunk iterations of the preconditioned BiCGStab for every scenario; seq
+// numbers the iterations launched in this solve (the finalize that stops a
+// scenario records it, so that iteration's x and r update alone still runs)
 template <bool OFF>
-static int bh_iterate(BatchHost* b, const BatchDev& B, int grow, int chunk) {
+static int bh_iterate(BatchHost* b, const BatchDev& B, int grow, int chunk, int& seq) {
   isph_ctx* c = b->ctx[0];
   const int S = b->S;
   int rc = 0;
-  for (int k = 0; k < chunk; k++) {
+  for (int k = 0; k < chunk; k++, seq++) {
     rc = bh_vcycle<0, OFF>(b, B, grow);
     if (rc) return rc;
     TIMED_BLAUNCH(2, kb_pb_1<OFF>, grow, B);
-    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_ALPHA, 0);
+    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_ALPHA, 0, seq);
     BLAUNCH(kb_pb_s, grow, 256, B);
     rc = bh_vcycle<1, OFF>(b, B, grow);
     if (rc) return rc;
     TIMED_BLAUNCH(3, kb_pb_2<OFF>, grow, B);
-    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_OMEGA, 0);
-    BLAUNCH(kb_pb_3, grow, 256, B);
-    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_BETA, 0);
-    BLAUNCH(kb_pb_4, grow, 256, B);
+    TIMED_BLAUNCH(9, kb_pfinalize, S, B, (int)PFIN_OMEGA, 0, seq);
+    BLAUNCH(kb_pb_34, grow, 256, B, seq);
   }
   return 0;
 }
@@ -1064,9 +1064,9 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
   BLAUNCH(kb_mg_dense, S, MG_CO_THREADS, B);
   // ||b||, then the FP64 start residual
   BLAUNCH(kb_pb_bnorm, grow, 256, B);
-  BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BNORM, 0);
+  BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_BNORM, 0, 0);
   BLAUNCH(kb_pb_resid, grow, 256, B, 1);
-  BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 1);
+  BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 1, 0);
   rc = bh_gather(b);
   if (rc) return rc;
   // error bits of the phases before the solve (neighbour capacity, PPE
@@ -1105,7 +1105,7 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
       if (!b->sc[s].fin) p = std::max(p, after ? b->ctx[s]->pred_after : b->ctx[s]->pred_first);
     return p;
   };
-  int chunk = 4;
+  int chunk = 4, seq = 0;
   bool guessed = false;  // the chunk just run was sized from the last solve
   if (const int p = predicted(false); p > 0) {
     chunk = p;
@@ -1116,11 +1116,12 @@ static int bh_solve(BatchHost* b, isph_solve_stats* stats) {
       set_err(c, "precision solve: scenarios did not finish");
       return ISPH_ENOCONV;
     }
-    rc = off16 ? bh_iterate<true>(b, B, grow, chunk) : bh_iterate<false>(b, B, grow, chunk);
+    rc = off16 ? bh_iterate<true>(b, B, grow, chunk, seq)
+               : bh_iterate<false>(b, B, grow, chunk, seq);
     if (rc) return rc;
     // FP64 check of every scenario whose iteration stopped in this chunk
     BLAUNCH(kb_pb_resid, grow, 256, B, 2);
-    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 2);
+    BLAUNCH(kb_pfinalize, S, 256, B, (int)PFIN_RESID, 2, 0);
     rc = bh_gather(b);
     if (rc) return rc;
     // the timed launches that did work: the chunk's leading iterations

@@ -352,7 +352,7 @@ __device__ __forceinline__ void mg_dense_block(const MgDev& G) {
 // V-cycle
 // ---------------------------------------------------------------------
 // level 0, row i: r0 = v - A x0, where x0 = (w v) d was written for every row
-// by the kernel that produced v (pb_s for s, pb_4 and pb_resid for p)
+// by the kernel that produced v (pb_s for s, pb_34 and pb_resid for p)
 template <bool OFF, typename VT>
 __device__ __forceinline__ void mg_pre0_row(const Dev& D, const MgDev& G,
                                             const VT* __restrict__ v, float* __restrict__ r0,
@@ -813,58 +813,52 @@ __device__ __forceinline__ void pb_s_rows(const Dev& D, const MgDev& G, int vb) 
   mg_scale0(G, (float)sv, i);
 }
 
-// t2 = A sh (FP64) ; partials (t2 . s), (t2 . t2)
+// t2 = A sh (FP64) ; partials (t2 . s), (t2 . t2), (t2 . rh), (s . rh),
+// (s . s) at k * nblk + vb, k = 0..4: from them the finalize forms omega and,
+// for r = s - omega t2, r . rh and ||r||^2, so the iteration needs no
+// further reduction over the rows
 template <bool OFF>
 __device__ __forceinline__ void pb_2_rows(const Dev& D, int vb, double* lds) {
   const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
   if (dn) return;
   const int i = vb * RED_BLOCK + threadIdx.x;
-  double p1 = 0.0, p2 = 0.0;
+  double q[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     const float* __restrict__ sh = D.sv32;
     const double acc = ell_row_sum_any<OFF, ELL_UP, double>(D, D.aval, i,
                                                  [&](int j) { return (double)sh[j]; });
     D.t2[i] = acc;
-    p1 = acc * D.sv[i];
-    p2 = acc * acc;
+    const double sv = D.sv[i], rh = (double)D.rh32[i];
+    q[0] = acc * sv;
+    q[1] = acc * acc;
+    q[2] = acc * rh;
+    q[3] = sv * rh;
+    q[4] = sv * sv;
   }
-  const double s1 = block_tree(p1, lds);
-  const double s2 = block_tree(p2, lds);
+  block_tree_n<5>(q, lds);
   if (threadIdx.x == 0) {
-    D.part[vb] = s1;
-    D.part[D.nblk + vb] = s2;
+#pragma unroll
+    for (int k = 0; k < 5; k++) D.part[k * D.nblk + vb] = q[k];
   }
 }
 
-// x += alpha ph + omega sh (FP64) ; r = s - omega t2 ; partials (r . rh), (r . r)
-__device__ __forceinline__ void pb_3_rows(const Dev& D, int vb, double* lds) {
-  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
-  const double alpha = ld_global(&D.sc->alpha), omega = ld_global(&D.sc->omega);
+// x += alpha ph + omega sh (FP64) ; r = s - omega t2 ; p = r + beta (p - omega t1)
+// (p stored FP32, with x0 = (w p) d for the next V-cycle).  seq numbers the
+// iteration's launches within the solve: the iteration whose finalize
+// stopped (stop_seq == seq) still takes its x and r update, and no other
+// after it.
+__device__ __forceinline__ void pb_34_rows(const Dev& D, const MgDev& G, int vb, int seq) {
+  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n),
+            sq = ld_global(&D.sc->stop_seq);
+  const double alpha = ld_global(&D.sc->alpha), omega = ld_global(&D.sc->omega),
+               beta = ld_global(&D.sc->beta);
+  const int i = vb * RED_BLOCK + threadIdx.x;
+  if (((dn != 0) & (sq != seq)) | (i >= n)) return;
+  D.sol[i] = (D.sol[i] + alpha * (double)D.r32[i]) + omega * (double)D.sv32[i];
+  const double r = D.sv[i] - omega * D.t2[i];
+  D.r[i] = r;
   if (dn) return;
-  const int i = vb * RED_BLOCK + threadIdx.x;
-  double p1 = 0.0, p2 = 0.0;
-  if (i < n) {
-    D.sol[i] = (D.sol[i] + alpha * (double)D.r32[i]) + omega * (double)D.sv32[i];
-    const double r = D.sv[i] - omega * D.t2[i];
-    D.r[i] = r;
-    p1 = r * (double)D.rh32[i];
-    p2 = r * r;
-  }
-  const double s1 = block_tree(p1, lds);
-  const double s2 = block_tree(p2, lds);
-  if (threadIdx.x == 0) {
-    D.part[vb] = s1;
-    D.part[D.nblk + vb] = s2;
-  }
-}
-
-// p = r + beta (p - omega t1)  (p stored FP32)
-__device__ __forceinline__ void pb_4_rows(const Dev& D, const MgDev& G, int vb) {
-  const int dn = ld_global(&D.sc->done), n = ld_global(&D.sc->n);
-  const double beta = ld_global(&D.sc->beta), omega = ld_global(&D.sc->omega);
-  const int i = vb * RED_BLOCK + threadIdx.x;
-  if ((dn != 0) | (i >= n)) return;  // both reads in one trip
-  const float p = (float)(D.r[i] + beta * ((double)D.pp32[i] - omega * D.t1[i]));
+  const float p = (float)(r + beta * ((double)D.pp32[i] - omega * D.t1[i]));
   D.pp32[i] = p;
   mg_scale0(G, p, i);
 }
@@ -926,31 +920,36 @@ __device__ __forceinline__ void pb_bnorm_rows(const Dev& D, int vb, double* lds)
 // one-workgroup finalize per scenario over its row blocks' partials.
 //   PFIN_BNORM  bnorm = sqrt(b . b); a zero or non-finite norm finishes the
 //               scenario (converged with x = 0 for a zero right-hand side)
-//   PFIN_ALPHA / PFIN_OMEGA / PFIN_BETA  the BiCGStab scalars; PFIN_BETA
-//               stops on the recursive residual at frac * tol only as a
-//               candidate (frac = inner_tol, PB_STOP_FRAC when unset), which
-//               the FP64 check confirms.  With
-//               the outer products in FP64 the recursive residual follows the
-//               true one, and at 650k the first check passes: 38 iterations
-//               per solve, where FP32 outer products needed 57 and a restart.
+//   PFIN_ALPHA  alpha = rho / (t1 . rh)
+//   PFIN_OMEGA  omega = (t2 . s) / (t2 . t2), and for r = s - omega t2 the
+//               next rho = s . rh - omega (t2 . rh), ||r||^2 = s . s - omega
+//               (t2 . s) (the minimum over omega) and beta.  It stops on
+//               ||r|| <= frac * tol * ||b|| only as a candidate (frac =
+//               inner_tol, PB_STOP_FRAC when unset), which the FP64 check
+//               confirms; a stop records seq, the iteration whose x and r
+//               update still runs.  With the outer products in FP64 the
+//               recursive residual follows the true one, and at 650k the
+//               first check passes: 38 iterations per solve, where FP32
+//               outer products needed 57 and a restart.
 //   PFIN_RESID  init 1: start state from the FP64 residual;
 //               init 2: check of a stopped iteration against the FP64
 //               residual: met -> final; else, unless the iteration or restart
 //               cap is reached, restart BiCGStab from the FP64 residual
 // ---------------------------------------------------------------------
-enum { PFIN_ALPHA = 0, PFIN_OMEGA = 1, PFIN_BETA = 2, PFIN_RESID = 3, PFIN_BNORM = 4 };
+enum { PFIN_ALPHA = 0, PFIN_OMEGA = 1, PFIN_RESID = 2, PFIN_BNORM = 3 };
 
-__device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, double* lds) {
+__device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, int seq,
+                                            double* lds) {
   DevScalars* S = D.sc;
-  if ((op == PFIN_ALPHA || op == PFIN_OMEGA || op == PFIN_BETA) && S->done) return;
+  if ((op == PFIN_ALPHA || op == PFIN_OMEGA) && S->done) return;
   if (op == PFIN_RESID && init == 1 && S->fin) return;
   if (op == PFIN_RESID && init == 2 && !(S->done && !S->fin)) return;
   const int nb = D.nblk;
-  double s0, s1;
-  final_tree2(D.part,
-              (op == PFIN_OMEGA || op == PFIN_BETA || op == PFIN_RESID) ? D.part + nb : nullptr,
-              nb, lds, s0, s1);
+  double s[5];
+  if (op == PFIN_OMEGA) final_tree_n<5>(D.part, nb, nb, lds, s);
+  else final_tree2(D.part, (op == PFIN_RESID) ? D.part + nb : nullptr, nb, lds, s[0], s[1]);
   if (threadIdx.x != 0) return;
+  const double s0 = s[0], s1 = s[1];
   const DevParams& P = D.P;
   switch (op) {
     case PFIN_BNORM: {
@@ -963,6 +962,7 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, doub
       S->conv = 0;
       S->fin = 0;
       S->rel = 0.0;
+      S->stop_seq = -1;
       if (bn == 0.0 || !isfinite(bn)) {
         S->fin = 1;
         S->done = 1;
@@ -975,12 +975,16 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, doub
       S->alpha = S->rho / s0;
       if (!isfinite(S->alpha)) { S->brk = 1; S->done = 1; }
       break;
-    case PFIN_OMEGA:
-      S->omega = s0 / s1;
-      if (!isfinite(S->omega) || S->omega == 0.0) { S->brk = 1; S->done = 1; }
-      break;
-    case PFIN_BETA: {
-      const double rhon = s0, rr = s1;
+    case PFIN_OMEGA: {
+      // s = (t2 . s, t2 . t2, t2 . rh, s . rh, s . s)
+      const double om = s0 / s1;
+      S->omega = om;
+      if (!isfinite(om) || om == 0.0) {
+        S->brk = 1;
+        S->done = 1;
+        break;
+      }
+      const double rr = fmax(s[4] - om * s0, 0.0), rhon = s[3] - om * s[2];
       S->iters = S->iters + 1;
       S->rr = rr;
       S->rel = sqrt(rr) / S->bnorm;
@@ -988,13 +992,18 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, doub
       if (sqrt(rr) <= frac * P.tol * S->bnorm) {
         S->conv = 1;  // candidate: confirmed against the FP64 residual (PFIN_RESID, init 2)
         S->done = 1;
+        S->stop_seq = seq;
       } else if (rhon == 0.0 || !isfinite(rhon)) {
         S->brk = 1;
         S->done = 1;
+        S->stop_seq = seq;
       } else {
-        S->beta = (rhon / S->rho) * (S->alpha / S->omega);
+        S->beta = (rhon / S->rho) * (S->alpha / om);
         S->rho = rhon;
-        if (S->iters >= P.max_iter) S->done = 1;
+        if (S->iters >= P.max_iter) {
+          S->done = 1;
+          S->stop_seq = seq;
+        }
       }
       break;
     }

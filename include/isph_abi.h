@@ -318,7 +318,7 @@ int isph_batch_autotune(isph_batch *b, int32_t trials, int32_t *shape, double *u
  * V-cycle's level-0 passes), "mg_restrict0" (the restriction to level 1),
  * "mg_down", "mg_up" (the multi-workgroup cell levels, every level),
  * "mg_coarse" (its single-workgroup coarse levels) and "pfinalize" (the
- * iteration's three dot-product finalizes) of the precision form. */
+ * iteration's two dot-product finalizes) of the precision form. */
 int isph_prof_enable(isph_ctx *ctx, int32_t on);
 int isph_prof_read(isph_ctx *ctx, const char *kernel, double *total_ms,
                    int64_t *launches);

@@ -2,13 +2,13 @@
 launches, and the counts follow the iteration's launch sequence (per
 iteration: one of each SpMV, two of each level-0 pass and of the level-1
 restriction and coarse workgroup, 2 x the multi-workgroup cell levels of
-each cell pass, three finalizes), counted over the iterations that did work."""
+each cell pass, two finalizes), counted over the iterations that did work."""
 import pytest
 
 TIMED = ("spmv_p32", "spmv_s32", "mg_pre0", "mg_post0", "mg_restrict0", "mg_down",
          "mg_coarse", "mg_up", "pfinalize")
 PER_ITER = {"spmv_p32": 1, "spmv_s32": 1, "mg_pre0": 2, "mg_post0": 2, "mg_restrict0": 2,
-            "mg_coarse": 2, "pfinalize": 3}
+            "mg_coarse": 2, "pfinalize": 2}
 
 
 def _multi_workgroup_levels(case):
