@@ -11,8 +11,8 @@ the 16-bit offset j - i in the precision form) and the FP64 value.
   k_bicg_2  t2 = A s, s = r - alpha t1 on the fly, partials (t2 . s), (t2 . t2)
                                              reads alen, r, t1; writes s, t2
   kb_pb_1   t1 = A ph, partial (t1 . rh32)    reads alen, ph (FP32), rh32; writes t1 (FP64)
-  kb_pb_2   t2 = A sh, partials (t2 . s), (t2 . t2)
-                                             reads alen, sh (FP32), s (FP64); writes t2 (FP64)
+  kb_pb_2   t2 = A sh, partials (t2 . s), (t2 . t2), (t2 . rh32), (s . rh32), (s . s)
+                                             reads alen, sh (FP32), s (FP64), rh32; writes t2 (FP64)
   kb_mg_pre0   r0 = v - A16 x0 (level 0 of the V-cycle, FP16 values, 16-bit
                offsets)                      reads alen, v (FP32 or FP64: 6 on average), x0 (own);
                                              writes r0
@@ -30,7 +30,7 @@ the 16-bit offset j - i in the precision form) and the FP64 value.
                1/diagonal and the parent's correction, writes the result; on
                level 1 also x0 of every row, read and written (prolongation)
   kb_pfinalize one workgroup folds the row blocks' FP64 partials (one array
-               for alpha, two for omega and beta)
+               for alpha, five for omega, the next rho and ||r||)
 """
 from __future__ import annotations
 
@@ -38,7 +38,7 @@ _PER_ROW = {
     "k_bicg_1": (4 + 8 + 8, 8),
     "k_bicg_2": (4 + 8 + 8, 16),
     "kb_pb_1": (4 + 4 + 4, 8),
-    "kb_pb_2": (4 + 4 + 8, 8),
+    "kb_pb_2": (4 + 4 + 8 + 4, 8),
     "kb_mg_pre0": (4 + 6 + 4, 4),
     "kb_mg_post0": (4 + 6 + 4 + 4, 4),
 }
@@ -75,13 +75,13 @@ def cell_pass_bytes(kernel: str, n: int, cellx: int, celly: int) -> float:
     """Algorithmic bytes of one launch of a cell-level kernel of the V-cycle
     at n rows: kb_mg_restrict0, or kb_mg_down / kb_mg_up averaged over the
     multi-workgroup levels (those before the coarse workgroup's first level),
-    or kb_pfinalize averaged over its three launches per iteration."""
+    or kb_pfinalize averaged over its two launches per iteration."""
     lv = cell_levels(cellx, celly)
     k = next(i for i, m in enumerate(lv) if m <= MG_CO_UNKNOWNS)
     if kernel == "kb_mg_restrict0":
         return 4.0 * n + 4.0 * (lv[0] + 1) + 4.0 * lv[0]
     if kernel == "kb_pfinalize":
-        return 8.0 * ((n + 255) // 256) * (1 + 2 + 2) / 3.0
+        return 8.0 * ((n + 255) // 256) * (1 + 5) / 2.0
     if k == 0:
         return 0.0
     if kernel == "kb_mg_down":
@@ -133,9 +133,11 @@ _STEP_PER_PARTICLE = {
 # Per iteration and row.  Plain: the two SpMVs with their dots, the x/r
 # update and the direction update.  Precision form: the two FP64-product
 # SpMVs, the four level-0 V-cycle passes (FP16 values and 16-bit offsets,
-# 4 B per entry, with their vectors), and the s, x/r and direction updates;
-# the cell levels are a few percent of the rows and are left out.
-_ITER_PER_ROW = {"plain": 64 + 32, "precision": 4 * 16 + 24 + 60 + 24}
+# 4 B per entry, with their vectors), the s update (reads r, t1, d0; writes
+# s, x0: 32 B) and the one x/r/direction pass (reads x, ph, sh, s, t2, p,
+# t1, d0; writes x, r, p, x0: 72 B); the cell levels are a few percent of
+# the rows and are left out.
+_ITER_PER_ROW = {"plain": 64 + 32, "precision": 4 * 16 + 32 + 72}
 _ITER_PER_NNZ = {"plain": 0, "precision": 4 * 4}
 
 
