@@ -247,6 +247,11 @@ def test_precision_form_residual_and_pressure_error(pkg, oracle, cases):
         print(f"step {k}: {st.iterations} iterations, {st.outer} restarts, "
               f"rel residual {st.rel_residual:.2e}")
         assert st.converged and st.rel_residual <= 1e-8
+        # the recursion stops at 0.05 tol and the returned x's FP64 residual is
+        # there too: the stopping iteration's own update was applied (without
+        # it x would hold the previous iterate, whose residual was above the
+        # stop), and the check needed no restart
+        assert st.outer == 0 and st.rel_residual <= 0.05 * 1e-8 * 1.02
     g.ppe_assemble()
     ln, col, val, rhs = g.ppe()
     x0 = g.solution()
@@ -260,6 +265,31 @@ def test_precision_form_residual_and_pressure_error(pkg, oracle, cases):
     print(f"precision form: {st.iterations} iterations, rel residual {st.rel_residual:.2e}; "
           f"FP64 reference {st64.iterations} iterations; pressure error {rel:.2e}")
     assert rel <= 1e-6  # the precision form's pressure target
+
+
+def test_precision_stop_is_independent_of_the_chunk(pkg, cases):
+    """The iteration at which the precision form stops applies its update once,
+    however many iterations the host launched after it in the same chunk: one
+    capped solve run with a chunk sized from the step's solve (~80 iterations,
+    75 of them launched after the stop) and again with a chunk of exactly its
+    5 (sized from the first solve) gives the same solution bit for bit."""
+    c = cases.dam_break(jitter=0.01)
+    g = pkg.Scenario(c, params=dict(c.params, precision=1, tol=1e-8))
+    g.init_state()
+    g.step(1)
+    g.ppe_assemble()
+    ln, col, val, rhs = g.ppe()
+    g.set_params(tol=1e-30, max_iter=5, max_outer=0)
+    sols = []
+    for k in range(2):
+        g.load_system(ln, col, val, rhs, x0=np.zeros(len(rhs)))
+        st = g.ppe_solve(check=False)
+        print(f"capped solve {k}: {st.iterations} iterations, rel residual "
+              f"{st.rel_residual:.3e}", flush=True)
+        assert st.iterations == 5 and not st.converged
+        sols.append(g.solution())
+    g.close()
+    assert np.array_equal(sols[0], sols[1])
 
 
 def test_precision_int32_column_path_matches_offsets(pkg, cases):
