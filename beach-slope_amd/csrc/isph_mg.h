@@ -253,25 +253,33 @@ __device__ __forceinline__ void mg_coarsen_cell(const MgDev& G, int l, int I) {
   if (I >= C.n) return;
   const int CJ = I / C.ny, CI = I - CJ * C.ny;
   const float* sf = G.sten + 9 * F.off;
+  // every child's nine stencil values are requested before the first sum (a
+  // child outside the fine grid reads cell 0; a term whose neighbour lies
+  // outside is not added), then summed in child and k order
+  float v[4][9];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int ci = 2 * CI + (c & 1), cj = 2 * CJ + (c >> 1);
+    const int f = (ci < F.ny && cj < F.nx) ? cj * F.ny + ci : 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[c][k] = sf[k * F.n + f];
+  }
   float s[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) s[k] = 0.0f;
-  for (int b = 0; b < 2; b++) {
-    for (int a = 0; a < 2; a++) {
-      const int ci = 2 * CI + a, cj = 2 * CJ + b;
-      if (ci >= F.ny || cj >= F.nx) continue;
-      const int f = cj * F.ny + ci;
 #pragma unroll
-      for (int k = 0; k < 9; k++) {
-        const int di = (k % 3) - 1, dj = (k / 3) - 1;
-        const int ni = ci + di, nj = cj + dj;
-        if (ni < 0 || ni >= F.ny || nj < 0 || nj >= F.nx) continue;
-        const int pi = (ni >> 1) - CI, pj = (nj >> 1) - CJ;
-        const float v = sf[k * F.n + f];
+  for (int c = 0; c < 4; c++) {
+    const int ci = 2 * CI + (c & 1), cj = 2 * CJ + (c >> 1);
+    const bool in = ci < F.ny && cj < F.nx;
 #pragma unroll
-        for (int q = 0; q < 9; q++)
-          if (q == (pj + 1) * 3 + (pi + 1)) s[q] += v;
-      }
+    for (int k = 0; k < 9; k++) {
+      const int di = (k % 3) - 1, dj = (k / 3) - 1;
+      const int ni = ci + di, nj = cj + dj;
+      const bool ok = in && ni >= 0 && ni < F.ny && nj >= 0 && nj < F.nx;
+      const int pi = (ni >> 1) - CI, pj = (nj >> 1) - CJ;
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        if (ok && q == (pj + 1) * 3 + (pi + 1)) s[q] += v[c][k];
     }
   }
   if (s[4] == 0.0f) s[4] = 1.0f;
@@ -284,12 +292,13 @@ __device__ __forceinline__ void mg_coarsen_cell(const MgDev& G, int l, int I) {
 // Coarsest level (one workgroup): dense matrix and its explicit inverse by
 // Gauss-Jordan with partial pivoting on the augmented matrix [A | I], held in
 // LDS (MG_DENSE_MAX x 2 MG_DENSE_MAX doubles = 64 KB).  Per pivot column: one
-// wavefront finds the pivot (largest magnitude, first row on ties), the rows
-// are swapped, the pivot row scaled, and all other rows updated in parallel
+// wavefront finds the pivot (largest magnitude, first row on ties), the swap
+// is recorded, the pivot row scaled, and all other rows updated in parallel
 // from snapshots of the pivot row and column.
 __device__ __forceinline__ void mg_dense_block(const MgDev& G) {
   __shared__ double A[MG_DENSE_MAX * 2 * MG_DENSE_MAX];
   __shared__ double colc[MG_DENSE_MAX], prow[2 * MG_DENSE_MAX];
+  __shared__ int perm[MG_DENSE_MAX];  // row r of the swapped matrix is stored row perm[r]
   __shared__ int piv;
   const MgLevel L = G.lev[G.nlev];
   const int n = L.n, t = threadIdx.x, T = blockDim.x;
@@ -298,45 +307,52 @@ __device__ __forceinline__ void mg_dense_block(const MgDev& G) {
   for (int e = t; e < n * w2; e += T) A[e] = 0.0;
   __syncthreads();
   for (int I = t; I < n; I += T) {
+    float v[9];  // every load before the first sum
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = st[k * n + I];
     for (int k = 0; k < 9; k++) {
       const int J = mg_nbr(L, I, k);
-      if (J >= 0) A[I * w2 + J] += (double)st[k * n + I];
+      if (J >= 0) A[I * w2 + J] += (double)v[k];
     }
     A[I * w2 + n + I] = 1.0;
+    perm[I] = I;
   }
   __syncthreads();
+  // Row swaps are recorded in perm instead of moving the rows: the pivot
+  // search, its tie rule and every update are those of the swapped matrix,
+  // so the values are the same, with one barrier and one pass fewer per column.
   for (int c = 0; c < n; c++) {
     if (t < 64) {
-      // lane r holds |A(r, c)| for r >= c (n <= 64 rows)
-      double bv = (t >= c && t < n) ? fabs(A[t * w2 + c]) : -1.0;
+      // lane r holds |A(r, c)| of swapped row r >= c (n <= 64 rows)
+      const bool cand = t >= c && t < n;
+      const int ph = cand ? perm[t] : 0;
+      double bv = cand ? fabs(A[ph * w2 + c]) : -1.0;
       int br = t;
       for (int off = 32; off >= 1; off >>= 1) {
         const double ov = __shfl_down(bv, off, 64);
         const int orow = __shfl_down(br, off, 64);
         if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
       }
-      if (t == 0) piv = br;
+      if (t == 0) {
+        const int pc = perm[c], pb = perm[br];
+        perm[c] = pb;
+        perm[br] = pc;
+        piv = pb;
+      }
     }
     __syncthreads();
-    const int pr = piv;
-    if (pr != c)
-      for (int e = t; e < w2; e += T) {
-        const double tmp = A[c * w2 + e];
-        A[c * w2 + e] = A[pr * w2 + e];
-        A[pr * w2 + e] = tmp;
-      }
-    __syncthreads();
-    const double pv = A[c * w2 + c];
-    for (int e = t; e < w2; e += T) prow[e] = (pv != 0.0) ? A[c * w2 + e] / pv : 0.0;
+    const int pr = piv;  // stored row of the pivot
+    const double pv = A[pr * w2 + c];
+    for (int e = t; e < w2; e += T) prow[e] = (pv != 0.0) ? A[pr * w2 + e] / pv : 0.0;
     for (int r = t; r < n; r += T) colc[r] = A[r * w2 + c];
     __syncthreads();
-    // thread t updates column t mod 2 MG_DENSE_MAX of rows t / (2 MG_DENSE_MAX),
+    // thread t updates column t mod 2 MG_DENSE_MAX of stored rows t / (2 MG_DENSE_MAX),
     // + T / (2 MG_DENSE_MAX), ... (no division in the loop)
     {
       const int q = t & (2 * MG_DENSE_MAX - 1), rs = T / (2 * MG_DENSE_MAX);
       if (q < w2)
         for (int r = t / (2 * MG_DENSE_MAX); r < n; r += rs)
-          A[r * w2 + q] = (r == c) ? prow[q] : A[r * w2 + q] - colc[r] * prow[q];
+          A[r * w2 + q] = (r == pr) ? prow[q] : A[r * w2 + q] - colc[r] * prow[q];
     }
     __syncthreads();
   }
@@ -344,7 +360,7 @@ __device__ __forceinline__ void mg_dense_block(const MgDev& G) {
   // threads (one per row r) read consecutive addresses
   for (int e = t; e < n * n; e += T) {
     const int cc = e / n, r = e - cc * n;
-    G.dense_inv[e] = A[r * w2 + n + cc];
+    G.dense_inv[e] = A[perm[r] * w2 + n + cc];
   }
 }
 
@@ -570,24 +586,28 @@ __device__ __forceinline__ void mg_coarse_block(const Dev& D, const MgDev& G,
   double dv[(MG_DENSE_MAX * MG_DENSE_MAX + MG_CO_THREADS - 1) / MG_CO_THREADS];
   float sv[MG_CO_STV];
   int ci0[MG_CO_PER], cj0[MG_CO_PER];
+  // A lane past an array's end reads its first element instead (every pool
+  // holds the dense level after these, so index 0 of each range is in
+  // bounds) and never uses the value: no load sits under a condition, where
+  // the compiler would give each its own wait.
 #pragma unroll
   for (int q = 0; q < MG_CO_PER; q++) {
-    const int I = t + q * T;
+    const int I = t + q * T, Ic = (I < L0.n) ? I : 0;
 #pragma unroll
-    for (int k = 0; k < 9; k++) st0[q][k] = (I < L0.n) ? sg0[k * L0.n + I] : 0.0f;
-    bq[q] = (I < L0.n) ? G.bb[base + I] : 0.0f;
-    rd0[q] = (I < L0.n) ? G.rdg[base + I] : 0.0f;
+    for (int k = 0; k < 9; k++) st0[q][k] = sg0[k * L0.n + Ic];
+    bq[q] = G.bb[base + Ic];
+    rd0[q] = G.rdg[base + Ic];
   }
-  const float rv = (t < nrd) ? G.rdg[stb + t] : 0.0f;
+  const float rv = G.rdg[stb + ((t < nrd) ? t : 0)];
 #pragma unroll
   for (int u = 0; u < (int)(sizeof dv / sizeof dv[0]); u++) {
     const int e = t + u * T;
-    dv[u] = (e < nd2) ? G.dense_inv[e] : 0.0;
+    dv[u] = G.dense_inv[(e < nd2) ? e : 0];
   }
 #pragma unroll
   for (int u = 0; u < MG_CO_STV; u++) {
     const int e = t + u * T;
-    sv[u] = (e < nst) ? sgs[e] : 0.0f;
+    sv[u] = sgs[(e < nst) ? e : 0];
   }
 #pragma unroll
   for (int q = 0; q < MG_CO_PER; q++) {
