@@ -1576,6 +1576,9 @@ static int step_pre(isph_ctx* c) {
   DevScalars s;
   int rc = read_scalars(c, &s);  // solsteps from the previous timestep
   if (rc) return rc;
+  // error bits the previous step's last phases set (its final check is left
+  // to this read when another step follows)
+  if (s.err && (rc = check_err_flags(c))) return rc;
   set_nhi(c, s.n);
   if (s.solsteps < 1 || s.solsteps > ISPH_MAX_SOLSTEPS) {
     set_err(c, "scalar sub-step count %d outside [1, %d]", s.solsteps, ISPH_MAX_SOLSTEPS);
@@ -1610,10 +1613,14 @@ static int step_pre(isph_ctx* c) {
   }
   rc = assemble_nocheck(c);
   if (rc) return rc;
+  // the precision-form solve reads the error bits with its first scalars and
+  // ends the step on them, so the GPU does not wait here for the host
+  if (c->P.precision != ISPH_PPE_FP64) return 0;
   return check_err_flags(c);
 }
 
-static int step_post(isph_ctx* c) {
+// check: read the error bits now (else the next step_pre's scalar read does)
+static int step_post(isph_ctx* c, bool check) {
   TLAUNCH(TK_COMPVEL, k_comp_vel, c->n_hi, c->D);
   LAUNCH(k_cell_max, c->ncell, c->D, c->ncell, 0);
   LAUNCH(k_comp_pos, c->n_hi, c->D);
@@ -1629,25 +1636,25 @@ static int step_post(isph_ctx* c) {
   rc = isph_timestep(c);
   if (rc) return rc;
   TLAUNCH(TK_EDDYVIS, k_eddyvis, c->n_hi, c->D);
-  return check_err_flags(c);
+  return check ? check_err_flags(c) : 0;
 }
 
-// a solve result that ends the step: errors other than non-convergence
-static bool solve_fatal(int rc) {
-  return rc == ISPH_ENONFINITE || rc == ISPH_EHIP || rc == ISPH_ESTATE || rc == ISPH_EARG;
-}
+// a solve result that ends the step: errors other than non-convergence (the
+// error bits of the phases before the solve, read by the precision form's
+// solve, included)
+static bool solve_fatal(int rc) { return rc != 0 && rc != ISPH_ENOCONV; }
 
-static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc) {
+static int step_once(isph_ctx* c, isph_solve_stats* stats, int* solve_rc, bool last) {
   int rc = step_pre(c);
   if (rc) return rc;
   if (c->wcsph) {
     if (stats) *stats = isph_solve_stats{};
-    return step_post(c);
+    return step_post(c, last);
   }
   rc = ppe_solve_impl(c, stats);
   if (solve_fatal(rc)) return rc;
   *solve_rc = rc;
-  return step_post(c);
+  return step_post(c, last);
 }
 
 extern "C" int isph_step(isph_ctx* c, int32_t nsteps, isph_solve_stats* stats) {
@@ -1655,7 +1662,7 @@ extern "C" int isph_step(isph_ctx* c, int32_t nsteps, isph_solve_stats* stats) {
   int worst = 0;
   for (int k = 0; k < nsteps; k++) {
     int srcode = 0;
-    int rc = step_once(c, stats, &srcode);
+    int rc = step_once(c, stats, &srcode, k == nsteps - 1);
     if (rc) return rc;
     if (srcode) worst = srcode;
   }
@@ -2181,7 +2188,7 @@ static int ph_step(isph_batch* b, std::vector<isph_solve_stats>& pst, int* worst
       src = check_err_flags(c);
       if (src == 0) src = ppe_solve_impl(c, &st);
     }
-    if (solve_fatal(src) || src == ISPH_ECAP) return src;
+    if (solve_fatal(src)) return src;
     if (src) *worst = src;
   }
   rc = ph_run(b, B, false);
@@ -2304,7 +2311,7 @@ extern "C" int isph_batch_step(isph_batch* b, int32_t nsteps) {
       if (rc) worst = rc;
     }
     for (isph_ctx* c : b->ctx) {
-      int rc = step_post(c);
+      int rc = step_post(c, true);
       if (rc) return rc;
     }
   }
