@@ -801,8 +801,8 @@ __device__ __forceinline__ void mg_post0_row(const Dev& D, const MgDev& G,
 // FP64, so the recursive residual follows the true one (b - A x changes by
 // exactly what the recursion subtracts, up to FP64 rounding) and the FP64
 // check at the stop passes without a restart.  Blocks write partial dot
-// products at vb and nblk + vb.  Every early return below depends only on
-// the scenario's scalars, so it is taken by whole workgroups.
+// products at k * nblk + vb for their k-th sum.  Every early return below
+// depends only on the scenario's scalars, so it is taken by whole workgroups.
 // ---------------------------------------------------------------------
 // t1 = A ph (FP64) ; partial (t1 . rh)
 template <bool OFF>
