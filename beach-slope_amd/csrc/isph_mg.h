@@ -1004,7 +1004,9 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, int 
         S->done = 1;
         break;
       }
-      const double rr = fmax(s[4] - om * s0, 0.0), rhon = s[3] - om * s[2];
+      // a rounding-negative ||r||^2 reads as 0; a non-finite one stays so
+      const double rr0 = s[4] - om * s0, rr = (rr0 < 0.0) ? 0.0 : rr0;
+      const double rhon = s[3] - om * s[2];
       S->iters = S->iters + 1;
       S->rr = rr;
       S->rel = sqrt(rr) / S->bnorm;
@@ -1013,7 +1015,7 @@ __device__ __forceinline__ void pb_finalize(const Dev& D, int op, int init, int 
         S->conv = 1;  // candidate: confirmed against the FP64 residual (PFIN_RESID, init 2)
         S->done = 1;
         S->stop_seq = seq;
-      } else if (rhon == 0.0 || !isfinite(rhon)) {
+      } else if (rhon == 0.0 || !isfinite(rhon) || !isfinite(rr)) {
         S->brk = 1;
         S->done = 1;
         S->stop_seq = seq;
