@@ -149,3 +149,21 @@ def test_neighbour_overflow_is_reported_and_cleared(pkg, oracle, cases):
     for i in range(len(cnt)):
         assert np.array_equal(lst[i, :cnt[i]], onb[i, :cnt[i]]), i
     s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "precision"])
+def test_neighbour_overflow_ends_the_step(pkg, cases, precision):
+    """A step whose neighbour search overflows ends with ECAP in both solve
+    forms: the FP64 form checks the error bits before its solve, the
+    precision form's solve reads them with its first scalars; in a two-step
+    call the first step ends it."""
+    c = _dense_cluster(cases)
+    s = pkg.Scenario(c, params=dict(c.params, precision=precision, tol=1e-8))
+    with pytest.raises(pkg.IsphError):
+        s.init_state()
+    for nsteps in (1, 2):
+        rc, _ = s.step(nsteps, check=False)
+        print(f"precision={precision} step({nsteps}) -> {rc}", flush=True)
+        assert rc == ECAP
+    s.close()
